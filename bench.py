@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Throughput of the plain GPU drop-in on its headline workload.
+
+Workload: 10M-agent synthetic household population (schools, classrooms,
+lognormal workplaces, shops, leisure, doctors, hospitals), 4 weeks at
+dt = 10 min (4032 steps), k = 3.74e-4, 0.1% initially infected, all policies
+on (DetailedTesting, quarantinePolicy 3, RuleClosure with the default
+closureRules.json, non-COVID hospitalisation, immunity waning).
+
+A "step" is one iteration of the reference's runSimulation loop: the
+midnight block when due (testing, progression, immunity, other disease,
+statistics, closures, location planning), then movement and infection for
+every agent.  --warmup steps run untimed first (default: day 0), then exactly
+--steps steps are timed between two device synchronisations (default: the
+remaining 27 days).  value = agents * steps / seconds (agent-steps/s).
+
+roofline: the per-step pipeline (mover scan, mover kernel, quarantine-write
+resolve, infection) is the dominant work; its algorithmic bytes per step are
+22 B/agent + 24 B/location + 4 B/relocated agent, divided by its device time
+from HIP events recorded around every step on the context stream during the
+timed region.  traffic: HBM bytes per step from rocprofv3 FETCH_SIZE /
+WRITE_SIZE when profiles/traffic_synth10M.json exists, else null.
+
+cpu_baseline: the oracle (sequential C++ re-statement of the reference
+per-step pipeline) on a bounded sample of the same workload (1M synthetic
+agents, one simulated day), on rank 0 only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3888)
+    ap.add_argument("--warmup", type=int, default=144)
+    ap.add_argument("--agents", type=int, default=10_000_000)
+    ap.add_argument("--weeks", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-agents", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=144)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(a):
+    from oracle.oracle import OracleSim
+    import pansim_amd
+
+    args = pansim_amd.synthetic_args(a.cpu_agents, 1, a.seed, 1)
+    o = OracleSim(args)
+    t0 = time.perf_counter()
+    o.steps(a.cpu_steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": o.n_agents * a.cpu_steps / dt,
+        "unit": "agent-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle (sequential C++ re-statement of the reference pipeline), {o.n_agents} synthetic agents, "
+                  f"{a.cpu_steps} steps incl. the day-0 midnight block, {dt:.1f} s",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (replicas would "
+                                   "shard whole populations across GPUs with no collectives)"}))
+        sys.exit(2)
+    import pansim_amd
+
+    total = a.weeks * 7 * 144
+    if a.warmup + a.steps > total:
+        a.steps = max(1, total - a.warmup)
+    args = pansim_amd.synthetic_args(a.agents, a.weeks, a.seed, 1)
+    t0 = time.perf_counter()
+    sim = pansim_amd.Simulation(args)
+    log(f"built {sim.n_agents} agents, {sim.n_locations} locations in {time.perf_counter() - t0:.1f}s")
+    sim.steps(a.warmup)
+    sim.sync()
+    log(f"warmup {a.warmup} steps done (t={sim.timestamp})")
+    sim.movement_counters(reset=True)
+    sim.enable_timing(True)
+    sim.sync()
+    t1 = time.perf_counter()
+    done = 0
+    chunk = 144
+    while done < a.steps:
+        n = min(chunk, a.steps - done)
+        sim.steps(n)
+        done += n
+        if done % (7 * 144) == 0:
+            log(f"  {done}/{a.steps} steps")
+    sim.sync()
+    elapsed = time.perf_counter() - t1
+    nm, move_ms, ni, infect_ms = sim.timing_summary()
+    evaluated, relocated = sim.movement_counters()
+    N, L = sim.n_agents, sim.n_locations
+    K = a.steps
+    value = N * K / elapsed
+    # algorithmic bytes of the per-step pipeline (movement + infection)
+    step_bytes_total = (22.0 * N + 24.0 * L) * K + 4.0 * relocated
+    pipeline_s = (move_ms + infect_ms) / 1e3
+    achieved = step_bytes_total / pipeline_s / 1e9 if pipeline_s > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_synth10M.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_step")
+        except Exception:
+            traffic = None
+    last = sim.day_stats(sim.num_days - 1)
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(a)
+    line = {
+        "metric": "agent-steps/s",
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+int64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain GPU drop-in, {N}-agent synthetic household population, {a.weeks} weeks at dt=10 min, "
+                        "k=3.74e-4, 0.1% infected at start, all policies on",
+            "n_agents": N,
+            "n_locations": L,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "step_pipeline": {
+            "movement_ms_per_step": move_ms / max(1, nm),
+            "infection_ms_per_step": infect_ms / max(1, ni),
+            "other_ms_per_step": (elapsed * 1e3 - move_ms - infect_ms) / K,
+            "movers_evaluated_per_step": evaluated / K,
+            "relocated_per_step": relocated / K,
+            "algorithmic_bytes_per_step": step_bytes_total / K,
+        },
+        "final_day_stats": last[:12],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,59 @@
+/*
+ * pansim_sim.h -- C ABI of the whole host simulation loop (the reference's
+ * Simulation<...> composition with the HIP-backed policies), for callers
+ * that drive PanSim as a library: the CLI, the Python package, and the
+ * calibration harness of the reference's MATLAB/pybind11 binding.
+ *
+ * pansim_sim_create + pansim_sim_run_for_day are the counterparts of
+ * SimulatorInterface::initSimulation / runForDay (matlab/mexPanSim.cpp:75-126,
+ * pybind11 module pyPanSim, :207-219).  pansim_sim_run is
+ * Simulation::runSimulation (include/simulation.h:899-926).
+ */
+#ifndef PANSIM_SIM_H
+#define PANSIM_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pansim_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pansim_sim pansim_sim;
+
+/* argv holds the reference's command-line flags without the program name
+ * ("-n", "100000", "--quarantinePolicy", "3", ...) plus --seed / --dataDir.
+ * print != 0 writes the statistics header and one line per day to stdout as
+ * the reference does.  On failure *out is NULL and pansim_sim_create_error()
+ * holds the message. */
+int pansim_sim_create(pansim_sim** out, int argc, const char* const* argv, int print);
+const char* pansim_sim_create_error(void);
+const char* pansim_sim_error(const pansim_sim* sim);
+void pansim_sim_destroy(pansim_sim* sim);
+
+/* n iterations of runSimulation's loop (midnight block when due, movement,
+ * closure step, infection, ++time). */
+int pansim_sim_steps(pansim_sim* sim, uint32_t n);
+/* Simulation::runSimulation: all weeks*7*stepsPerDay steps. */
+int pansim_sim_run(pansim_sim* sim);
+/* Simulation::runForDay with processFlags strings (TPdef, PL0, CF2000-0500,
+ * SO12, QU2, MA0.8, ...).  Writes the day's statistics vector. */
+int pansim_sim_run_for_day(pansim_sim* sim, int nflags, const char* const* flags, uint32_t* out, size_t cap, size_t* len);
+
+uint32_t pansim_sim_timestamp(const pansim_sim* sim);
+uint32_t pansim_sim_steps_per_day(const pansim_sim* sim);
+uint32_t pansim_sim_total_steps(const pansim_sim* sim);
+int pansim_sim_num_days(const pansim_sim* sim);
+/* Statistics vector of the d-th midnight (runSimulation) or runForDay call. */
+int pansim_sim_day_stats(const pansim_sim* sim, int day, uint32_t* out, size_t cap, size_t* len);
+/* Tab-separated statistics header (simulation.h:754). */
+int pansim_sim_header(const pansim_sim* sim, char* out, size_t cap);
+int pansim_sim_sizes(const pansim_sim* sim, uint32_t* n_agents, uint32_t* n_locations, uint32_t* n_variants);
+pansim_ctx* pansim_sim_context(pansim_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,148 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg).  ctypes front end of oracle/_build/liboracle.so,
+the sequential CPU re-statement of the reference per-timestep pipeline
+(oracle/oracle.cpp)."""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Sequence
+
+import numpy as np
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+DATA_DIR = os.path.join(os.path.dirname(ORACLE_DIR), "pansim_amd", "data")
+
+_lib = None
+LOCATION_FIELDS = ["locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness", "locCount"]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run pansim_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.oracle_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    L.oracle_create.restype = ctypes.c_void_p
+    L.oracle_last_error.restype = ctypes.c_char_p
+    L.oracle_destroy.argtypes = [ctypes.c_void_p]
+    L.oracle_steps.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_run_for_day.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), u32p, ctypes.c_int]
+    for f in ("oracle_timestamp", "oracle_num_agents", "oracle_num_locations"):
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+        getattr(L, f).restype = ctypes.c_uint32
+    L.oracle_num_days.argtypes = [ctypes.c_void_p]
+    L.oracle_day_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p, ctypes.c_int]
+    L.oracle_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, u32p]
+    L.oracle_location_sums.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_longlong)]
+    L.oracle_philox.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
+    L.oracle_expf.argtypes = [ctypes.c_float]
+    L.oracle_expf.restype = ctypes.c_float
+    L.oracle_log.argtypes = [ctypes.c_double]
+    L.oracle_log.restype = ctypes.c_double
+    L.oracle_seasonality.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.oracle_seasonality.restype = ctypes.c_double
+    L.oracle_length_p.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.oracle_length_p.restype = ctypes.c_double
+    L.oracle_geometric.argtypes = [ctypes.c_ulonglong, ctypes.c_uint32, ctypes.c_double]
+    L.oracle_geometric.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _argv(args: Sequence[str]):
+    arr = (ctypes.c_char_p * max(1, len(args)))()
+    for i, a in enumerate(args):
+        arr[i] = str(a).encode()
+    return arr
+
+
+class OracleSim:
+    def __init__(self, args: Sequence[str]):
+        args = [str(a) for a in args]
+        if "--dataDir" not in args:
+            args += ["--dataDir", DATA_DIR]
+        h = lib().oracle_create(len(args), _argv(args))
+        if not h:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        self._h = h
+        self.n_agents = lib().oracle_num_agents(h)
+        self.n_locations = lib().oracle_num_locations(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self, n: int):
+        if lib().oracle_steps(self._h, n) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+
+    def run_for_day(self, flags: Sequence[str] = ()) -> List[int]:
+        out = (ctypes.c_uint32 * 512)()
+        n = lib().oracle_run_for_day(self._h, len(flags), _argv(flags), out, 512)
+        if n < 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return list(out[:n])
+
+    @property
+    def timestamp(self) -> int:
+        return lib().oracle_timestamp(self._h)
+
+    @property
+    def num_days(self) -> int:
+        return lib().oracle_num_days(self._h)
+
+    def day_stats(self, d: int) -> List[int]:
+        out = (ctypes.c_uint32 * 512)()
+        n = lib().oracle_day_stats(self._h, d, out, 512)
+        return list(out[:n])
+
+    def field(self, name: str) -> np.ndarray:
+        n = self.n_locations if name in LOCATION_FIELDS else self.n_agents
+        out = np.zeros(n, dtype=np.uint32)
+        if lib().oracle_field(self._h, name.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))) != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return out
+
+    def location_sums(self, variant: int) -> np.ndarray:
+        out = np.zeros(self.n_locations, dtype=np.int64)
+        if lib().oracle_location_sums(self._h, variant, out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))) != 0:
+            raise RuntimeError("bad variant")
+        return out
+
+    def seasonality(self, ts: int) -> float:
+        return lib().oracle_seasonality(self._h, ts)
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    o = (ctypes.c_uint32 * 4)()
+    lib().oracle_philox(c, key[0], key[1], o)
+    return list(o)
+
+
+def expf(x: float) -> float:
+    return lib().oracle_expf(x)
+
+
+def log(x: float) -> float:
+    return lib().oracle_log(x)
+
+
+def length_p(avg: float, mx: float) -> float:
+    return lib().oracle_length_p(avg, mx)
+
+
+def geometric(seed: int, ident: int, p: float) -> int:
+    return lib().oracle_geometric(seed, ident, p)

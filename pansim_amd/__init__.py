@@ -1,0 +1,262 @@
+"""pansim_amd: MI355X-native PanSim per-timestep agent pipeline.
+
+Python front end of the native library (pansim_amd/_lib/libpansim.so, built
+by pansim_amd/build.py): the host Simulation<...> loop with HIP-backed
+policies, reached through the C ABI of include/pansim_sim.h and
+include/pansim_hip.h.  There is no Python or CPU fallback: if the library is
+missing, or no GPU is present, the calls raise.
+
+    sim = pansim_amd.Simulation(["-n", "100000", "-N", "20000", "-w", "2"])
+    sim.run()                      # Simulation::runSimulation
+    stats = sim.day_stats(3)       # per-day statistics vector
+
+    si = pansim_amd.SimulatorInterface()   # pyPanSim.SimulatorInterface
+    si.initSimulation([...]); si.runForDay(["TPdef", "QU3"])
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "_lib")
+DATA_DIR = os.path.join(PKG_DIR, "data")
+LIB_PATH = os.path.join(LIB_DIR, "libpansim.so")
+
+_lib = None
+
+AGENT_FIELDS = [
+    "state", "variant", "days", "location", "stepsUntilMove", "diagnosed", "quarantined", "stayedHome",
+    "noWorkAgent", "lastTest", "susceptible0", "infectedTimestamp", "infectedLocation", "worstStateTimestamp",
+    "worstStateEndTimestamp", "diagnosedTimestamp", "quarantinedTimestamp", "quarantinedUntilTimestamp",
+    "daysInQuarantine", "hospitalizedTimestamp", "hospitalizedUntilTimestamp", "recoveredTimestamp",
+    "infectedCount", "statsVariant", "worstState",
+]
+LOCATION_FIELDS = ["locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness", "locCount"]
+
+
+class PanSimError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpansim.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"native library missing: {LIB_PATH} (run pansim_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    c_u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.pansim_sim_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
+    L.pansim_sim_create_error.restype = ctypes.c_char_p
+    L.pansim_sim_error.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_error.restype = ctypes.c_char_p
+    L.pansim_sim_destroy.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_sim_run.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_run_for_day.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), c_u32p,
+                                         ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    for f in ("pansim_sim_timestamp", "pansim_sim_steps_per_day", "pansim_sim_total_steps"):
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+        getattr(L, f).restype = ctypes.c_uint32
+    L.pansim_sim_num_days.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_day_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, c_u32p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]
+    L.pansim_sim_header.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    L.pansim_sim_sizes.argtypes = [ctypes.c_void_p, c_u32p, c_u32p, c_u32p]
+    L.pansim_sim_context.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_context.restype = ctypes.c_void_p
+    L.pansim_download_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32, c_u32p, ctypes.c_size_t]
+    L.pansim_download_location_sums.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int64),
+                                                ctypes.c_size_t]
+    L.pansim_last_error.argtypes = [ctypes.c_void_p]
+    L.pansim_last_error.restype = ctypes.c_char_p
+    L.pansim_last_kernel_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.pansim_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_timing_summary.argtypes = [ctypes.c_void_p, c_u32p, ctypes.POINTER(ctypes.c_double), c_u32p,
+                                        ctypes.POINTER(ctypes.c_double)]
+    L.pansim_movement_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32]
+    L.pansim_sync.argtypes = [ctypes.c_void_p]
+    L.pansim_device_expf.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]
+    L.pansim_device_log.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
+    L.pansim_device_philox.argtypes = [c_u32p, ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_size_t]
+    _lib = L
+    return L
+
+
+def _argv(args: Sequence[str]):
+    arr = (ctypes.c_char_p * max(1, len(args)))()
+    for i, a in enumerate(args):
+        arr[i] = str(a).encode()
+    return arr
+
+
+def with_data_dir(args: Sequence[str]) -> List[str]:
+    """Adds --dataDir pointing at the shipped input files unless given."""
+    args = [str(a) for a in args]
+    if "--dataDir" not in args:
+        args += ["--dataDir", DATA_DIR]
+    return args
+
+
+class Simulation:
+    """The reference's Simulation<...> (runSimulation / runForDay) on the GPU."""
+
+    def __init__(self, args: Sequence[str], print_output: bool = False):
+        L = lib()
+        self._args = with_data_dir(args)
+        h = ctypes.c_void_p()
+        rc = L.pansim_sim_create(ctypes.byref(h), len(self._args), _argv(self._args), 1 if print_output else 0)
+        if rc != 0:
+            raise PanSimError(rc, L.pansim_sim_create_error().decode())
+        self._h = h
+        na, nl, nv = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        L.pansim_sim_sizes(self._h, ctypes.byref(na), ctypes.byref(nl), ctypes.byref(nv))
+        self.n_agents, self.n_locations, self.n_variants = na.value, nl.value, nv.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pansim_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_sim_error(self._h).decode())
+
+    @property
+    def ctx(self):
+        return lib().pansim_sim_context(self._h)
+
+    def steps(self, n: int):
+        self._check(lib().pansim_sim_steps(self._h, n))
+
+    def run(self):
+        self._check(lib().pansim_sim_run(self._h))
+
+    def run_for_day(self, flags: Sequence[str] = ()) -> List[int]:
+        out = (ctypes.c_uint32 * 512)()
+        n = ctypes.c_size_t()
+        self._check(lib().pansim_sim_run_for_day(self._h, len(flags), _argv(flags), out, 512, ctypes.byref(n)))
+        return list(out[: n.value])
+
+    @property
+    def timestamp(self) -> int:
+        return lib().pansim_sim_timestamp(self._h)
+
+    @property
+    def steps_per_day(self) -> int:
+        return lib().pansim_sim_steps_per_day(self._h)
+
+    @property
+    def total_steps(self) -> int:
+        return lib().pansim_sim_total_steps(self._h)
+
+    @property
+    def num_days(self) -> int:
+        return lib().pansim_sim_num_days(self._h)
+
+    def day_stats(self, d: int) -> List[int]:
+        out = (ctypes.c_uint32 * 512)()
+        n = ctypes.c_size_t()
+        self._check(lib().pansim_sim_day_stats(self._h, d, out, 512, ctypes.byref(n)))
+        return list(out[: n.value])
+
+    @property
+    def header(self) -> List[str]:
+        buf = ctypes.create_string_buffer(4096)
+        lib().pansim_sim_header(self._h, buf, 4096)
+        return [x for x in buf.value.decode().split("\t") if x]
+
+    def field(self, name: str) -> np.ndarray:
+        n = self.n_locations if name in LOCATION_FIELDS else self.n_agents
+        out = np.zeros(n, dtype=np.uint32)
+        rc = lib().pansim_download_field(self.ctx, name.encode(), self.timestamp,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
+        return out
+
+    def enable_timing(self, on: bool = True):
+        self._check_ctx(lib().pansim_enable_timing(self.ctx, 1 if on else 0))
+
+    def timing_summary(self):
+        """(n_movement_calls, movement_ms, n_infection_calls, infection_ms) since enable_timing."""
+        nm, ni = ctypes.c_uint32(), ctypes.c_uint32()
+        mm, mi = ctypes.c_double(), ctypes.c_double()
+        self._check_ctx(lib().pansim_timing_summary(self.ctx, ctypes.byref(nm), ctypes.byref(mm), ctypes.byref(ni),
+                                                    ctypes.byref(mi)))
+        return nm.value, mm.value, ni.value, mi.value
+
+    def movement_counters(self, reset: bool = False):
+        """(agents whose movement routine ran, agents that changed location) since the last reset."""
+        e, r = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check_ctx(lib().pansim_movement_counters(self.ctx, ctypes.byref(e), ctypes.byref(r), 1 if reset else 0))
+        return e.value, r.value
+
+    def sync(self):
+        self._check_ctx(lib().pansim_sync(self.ctx))
+
+    def _check_ctx(self, rc):
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
+
+    def location_sums(self, variant: int) -> np.ndarray:
+        out = np.zeros(self.n_locations, dtype=np.int64)
+        rc = lib().pansim_download_location_sums(self.ctx, variant, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                 self.n_locations)
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
+        return out
+
+
+class SimulatorInterface:
+    """Mirror of the reference's pybind11 pyPanSim.SimulatorInterface
+    (matlab/mexPanSim.cpp:57-127, 207-219)."""
+
+    def __init__(self):
+        self.sim = None
+
+    def initSimulation(self, args: Sequence[str]) -> None:
+        args = list(args)
+        if args and not str(args[0]).startswith("-"):
+            args = args[1:]  # the reference receives argv including the program name
+        self.sim = Simulation(args)
+
+    def runForDay(self, flags: Sequence[str] = ()) -> List[int]:
+        if self.sim is None:
+            raise RuntimeError("initSimulation was not called")
+        return self.sim.run_for_day(list(flags))
+
+
+def config1_args(n_agents=100_000, n_locations=20_000, weeks=2, seed=0, extra: Sequence[str] = ()) -> List[str]:
+    """The CPU-runnable reference workload: dummy population -n/-N, default
+    progression/parameters/agentTypes, quarantinePolicy 3, closures off,
+    0.1% initially in state E, dt = 10 min."""
+    return [
+        "-n", str(n_agents), "-N", str(n_locations), "-w", str(weeks), "-t", "10",
+        "--quarantinePolicy", "3", "--enableClosures", "0", "--closures", "inputConfigFiles/noRules.json",
+        "-c", "configs/configRandom_E0.001.json", "--seed", str(seed), *extra,
+    ]
+
+
+def synthetic_args(n_agents=10_000_000, weeks=4, seed=0, synth_seed=1, extra: Sequence[str] = ()) -> List[str]:
+    """Synthetic household population (schools, workplaces, shops, ...),
+    all policies on (testing, quarantine policy 3, closure rules)."""
+    return [
+        "--synthAgents", str(n_agents), "--synthSeed", str(synth_seed), "--synthInfected", "0.001",
+        "-w", str(weeks), "-t", "10", "-k", "0.000374", "--quarantinePolicy", "3",
+        "--closures", "inputConfigFiles/closureRules.json", "--seed", str(seed), *extra,
+    ]

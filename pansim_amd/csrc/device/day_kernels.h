@@ -1,0 +1,725 @@
+// Midnight kernels: initial state draws, occupancy rebuild, testing,
+// progression, immunity waning, non-COVID hospitalisation, statistics,
+// location planning, closure toggles.  Included once, by engine.hip.
+#pragma once
+#include "step_kernels.h"
+
+namespace ps {
+namespace dev {
+
+// ------------------------------------------------------------ common helpers
+__device__ __forceinline__ int geometric(double p, Stream& rs) {
+    double log1mp = ps_log(1.0 - p);
+    const double naf = (1.0 - 2.220446049250313e-16) / 2.0;
+    const double thr = 2147483647.0 + naf;
+    double cand;
+    int guard = 0;
+    do {
+        cand = floor(ps_log(1.0 - rs.unit()) / log1mp);
+    } while (cand >= thr && ++guard < 1000);
+    if (cand >= thr) cand = 2147483647.0;
+    return (int)(cand + naf);
+}
+
+// BasicLengthAbstract::LengthOfState::calculateDays (basicLengthAbstract.cpp:48-53)
+__device__ __forceinline__ int calc_days(const Tables& T, uint32_t prog, uint32_t state, uint32_t variant, Stream& rs) {
+    size_t k = ((size_t)prog * T.nStates + state) * 7 + variant;
+    if (T.lenAvg[k] == -1.0f) return -1;
+    double p = T.lenP[k];
+    float mx = T.lenMax[k];
+    int days = geometric(p, rs);
+    int guard = 0;
+    while (mx < (float)days && ++guard < 1000) days = geometric(p, rs);
+    if (mx < (float)days) days = (int)mx;
+    return days;
+}
+
+__device__ __forceinline__ float scaling_symptoms(const Agents& A, uint32_t i, uint32_t v, uint32_t s) {
+    // scalingSymptoms[v*7 + MAX(6, s)] (agentMeta.cpp:128-129): only slot 6 of
+    // each variant is ever written; other slots hold scalingAgeSex
+    return s <= 6 ? A.slot6[(size_t)v * A.n + i] : A.scal[i];
+}
+
+__device__ __forceinline__ void update_meta(const Agents& A, const Tables& T, uint32_t i, uint8_t st) {
+    float sf = T.stSusc[st] ? 1.0f : 0.0f;
+    for (uint32_t v = 0; v < T.nVar; ++v) A.susc[(size_t)v * A.n + i] = sf;
+}
+
+// ------------------------------------------------------------ init
+__global__ void k_init_agents(Agents A, Tables T, uint64_t seed, uint32_t replica, uint32_t qp, uint32_t qlenSteps,
+                              uint32_t qlenDays) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    uint8_t st = A.state[i];
+    A.variant[i] = 0;
+    update_meta(A, T, i, st);
+    for (uint32_t v = 0; v < T.nVar; ++v) A.slot6[(size_t)v * A.n + i] = A.scal[i];
+    Stream rs(seed, i, 0, PH_INIT_DAYS, replica);
+    A.days[i] = (int16_t)calc_days(T, A.prog[i], st, 0, rs);
+    bool inf = T.stInfected[st] != 0;
+    A.infTs[i] = inf ? 0u : NEVER;
+    A.infCount[i] = inf ? 1 : 0;
+    A.worstState[i] = inf ? (int8_t)st : 0;
+    A.worstTs[i] = 0;
+    A.worstEndTs[i] = 0;
+    A.infLoc[i] = 0;
+    A.recTs[i] = NEVER;
+    A.hospTs[i] = 0;
+    A.hospUntil[i] = 0;
+    A.immTs[i] = 0;
+    A.immCount[i] = 0;
+    A.statVar[i] = 0;
+    A.lastTest[i] = NEVER;
+    A.wake[i] = 0;
+    uint8_t fl = A.flags[i];  // diagnosed / hcw preset by the host upload
+    bool diag = fl & F_DIAGNOSED;
+    A.diagTs[i] = diag ? 1u : 0u;
+    if (diag && qp > 0) {
+        A.quarTs[i] = 0;
+        A.quarUntil[i] = qlenSteps;
+        A.daysInQ[i] = qlenDays;
+        fl |= F_QUARANTINED;
+    } else {
+        A.quarTs[i] = 0;
+        A.quarUntil[i] = 0;
+        A.daysInQ[i] = 0;
+    }
+    fl |= F_STAYED_HOME;
+    A.flags[i] = fl;
+}
+
+// ------------------------------------------------------------ occupancy
+__global__ void k_zero_locations(Locs L, uint32_t nVar) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
+        L.count[l] = 0u;
+        for (uint32_t v = 0; v < nVar; ++v) L.sum[(size_t)v * L.n + l] = 0;
+    }
+}
+
+__global__ void k_scatter_locations(Agents A, Locs L, Tables T) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    uint32_t loc = A.location[i];
+    atomicAdd(&L.count[loc], 1u);
+    float x = T.stInf[A.state[i]] * T.varMul[A.variant[i]];
+    if (x != 0.0f)
+        atomicAdd((unsigned long long*)&L.sum[(size_t)A.variant[i] * L.n + loc], (unsigned long long)fixed32(x));
+}
+
+__global__ void k_count_variants(Agents A, Tables T, unsigned long long* out) {
+    __shared__ unsigned int sh[MAXV];
+    if (threadIdx.x < MAXV) sh[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n && T.stInfected[A.state[i]]) atomicAdd(&sh[A.variant[i]], 1u);
+    __syncthreads();
+    if (threadIdx.x < T.nVar && sh[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ testing
+struct TestArgs {
+    Agents A;
+    Locs L;
+    Tables T;
+    uint64_t seed;
+    uint32_t replica, ts, dt, delay, qlen, usePCR;
+    double pRandom, pHome, pWork, pSchool, pHospital, pNursery;
+    unsigned long long* counters;  // tests, positive now, positive in the last day
+};
+
+__global__ void k_clear_flags(Locs L) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) L.flag[l] = 0;
+}
+
+// DetailedTestingOps::flagLocations (testingPolicies.h:59-148)
+__global__ void k_flag_locations(TestArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.A.n) return;
+    const uint32_t day = 24u * 60u / a.dt;
+    if (!(a.A.diagTs[i] > a.ts - day)) return;
+    const pansim_location_types& lt = a.T.lt;
+    const uint32_t ql = a.qlen * 24u * 60u / a.dt;
+    auto ok = [&](uint32_t l) {
+        uint32_t q = a.L.quarUntil[l];
+        return q == 0 || (q != 0 && (q - ql >= a.ts - day || q < a.ts));
+    };
+    uint32_t home = find_loc(a.A, a.L, lt, i, lt.home, 0, nullptr);
+    if (home != NOLOC) a.L.flag[home] = 1;
+    uint32_t work = find_loc(a.A, a.L, lt, i, lt.work, 0, nullptr);
+    if (work != NOLOC && ok(work)) a.L.flag[work] = 1;
+    uint32_t school = find_loc(a.A, a.L, lt, i, lt.school, 0, nullptr);
+    if (school != NOLOC && ok(school)) {
+        a.L.flag[school] = 1;
+        Stream rs(a.seed, i, a.ts, PH_TEST_FLAG, a.replica);
+        uint32_t cls = find_loc(a.A, a.L, lt, i, lt.classroom, 0, &rs);
+        if (cls != NOLOC && ok(cls)) a.L.flag[cls] = 1;
+    }
+}
+
+// DetailedTestingOps::doTesting (testingPolicies.h:157-281) + the counts of performTests
+__global__ void k_do_testing(TestArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned int sh[3];
+    if (threadIdx.x < 3) sh[threadIdx.x] = 0;
+    __syncthreads();
+    if (i < a.A.n) {
+        const Agents& A = a.A;
+        const uint32_t day = 24u * 60u / a.dt;
+        const uint32_t ts = a.ts;
+        const pansim_location_types& lt = a.T.lt;
+        uint8_t fl = A.flags[i];
+        bool skip = ts > a.delay * day && A.lastTest[i] != NEVER && A.lastTest[i] > ts - a.delay * day;
+        uint8_t st = A.state[i];
+        if (!skip && !(a.T.stWB[st] == 4 || (fl & F_DIAGNOSED))) {
+            Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
+            uint32_t home = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
+            bool homeFlag = home != NOLOC ? a.L.flag[home] != 0 : false;
+            uint32_t work = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
+            bool workFlag = work != NOLOC ? a.L.flag[work] != 0 : false;
+            uint32_t school = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
+            bool schoolFlag = false, classFlag = false;
+            if (school != NOLOC) {
+                schoolFlag = a.L.flag[school] != 0;
+                uint32_t cls = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
+                if (cls != NOLOC) classFlag = true;
+            }
+            double prob = a.pRandom;
+            prob += (double)homeFlag * a.pHome;
+            prob += (double)workFlag * a.pWork;
+            prob += (double)schoolFlag * a.pSchool;
+            prob += (double)classFlag * 3.0 * a.pSchool;
+            if (work != NOLOC && (a.L.type[work] == lt.doctor || a.L.type[work] == lt.hospital)) prob += a.pHospital;
+            if (work != NOLOC && a.L.type[work] == lt.nursery_home) prob += a.pNursery;
+            if (A.hospTs[i] <= ts && A.hospUntil[i] > ts) prob += a.pHospital;
+            if (prob > 1.0 || rs.real(1.0) < prob) {
+                A.lastTest[i] = ts;
+                if (a.T.stInfected[st]) {
+                    float pr = a.usePCR ? a.T.accPCR[st] : a.T.accAnt[st];
+                    if ((double)pr > rs.real(1.0)) {
+                        A.flags[i] = (uint8_t)(fl | F_DIAGNOSED);
+                        A.diagTs[i] = ts;
+                    }
+                } else {
+                    if (A.quarUntil[i] > ts && (home != NOLOC && a.L.quarUntil[home] < ts)) {
+                        if (A.daysInQ[i] > 0) A.daysInQ[i] -= (A.quarUntil[i] - ts) / day;
+                        A.quarUntil[i] = ts;
+                    }
+                }
+            }
+        }
+        if (A.lastTest[i] == ts) atomicAdd(&sh[0], 1u);
+        uint32_t d = A.diagTs[i];
+        if (d == ts) atomicAdd(&sh[1], 1u);
+        if (d < ts && d > ts - day) atomicAdd(&sh[2], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && sh[threadIdx.x]) atomicAdd(&a.counters[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ progression
+// Simulation::updateAgents (simulation.h:425-471), DynamicPPState::update
+// (dynamicPPState.cpp:246-303), NextStates::selectNext (multiBadMatrix.cpp:9-31)
+__global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replica, uint32_t ts, uint32_t dt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    int16_t days = A.days[i];
+    if (days > 0) {
+        A.days[i] = (int16_t)(days - 1);
+        return;
+    }
+    if (days != -2 && days != 0) return;
+    uint8_t st = A.state[i];
+    uint8_t v = A.variant[i];
+    uint32_t prog = A.prog[i];
+    float progS;
+    if (st == 7) progS = T.deathScaling[v] / scaling_symptoms(A, i, v, st);
+    else progS = T.progScaling[v];
+    float scaling = scaling_symptoms(A, i, v, st) * progS;
+    Stream rs(seed, i, ts, PH_PROGRESSION, replica);
+    if (days == -2) {
+        days = (int16_t)calc_days(T, prog, st, v, rs);
+        if (v >= 3) {
+            double d = days * 0.7;
+            days = (int16_t)(d == 0 ? 1.0 : d);
+        }
+    }
+    if (days == 0) {
+        uint8_t old = st;
+        bool wasInf = T.stInfected[st] != 0;
+        // selectNext
+        size_t ps_ = (size_t)prog * T.nStates + st;
+        uint32_t eb = T.edgeOff[ps_], nb = T.badCount[ps_], ee = T.edgeOff[ps_ + 1];
+        uint32_t nNeu = ee - eb - nb;
+        float sc = nNeu == 0 ? 1.0f : scaling;
+        double random = rs.unit();
+        double preSum = 0.0, badSum = 0.0;
+        uint32_t next = 0xFFFFFFFFu;
+        bool bad = false;
+        for (uint32_t j = 0; j < nb; ++j) {
+            float term = T.edgeChance[eb + j] * sc;
+            preSum += (double)term;
+            if (random < preSum) {
+                next = T.edgeTo[eb + j];
+                bad = true;
+                break;
+            }
+            badSum += (double)T.edgeChance[eb + j];
+        }
+        if (!bad && nNeu == 0) {  // bad chances summed below the draw
+            next = T.edgeTo[eb + nb - 1];
+            bad = true;
+        }
+        if (!bad) {
+            double badScaling = 1.0 + ((badSum - preSum) / (1.0 - badSum));
+            if (nb == 0) badScaling = 1.0;
+            uint32_t idx = 0;
+            do {
+                preSum += (double)T.edgeChance[eb + nb + idx] * badScaling;
+                ++idx;
+            } while (preSum < random && idx < nNeu);
+            idx--;
+            next = T.edgeTo[eb + nb + idx];
+        }
+        int nd = calc_days(T, prog, next, v, rs);
+        st = (uint8_t)next;
+        A.state[i] = st;
+        update_meta(A, T, i, st);
+        days = (int16_t)nd;
+        if (v >= 3 && st < 8) {
+            float mul = st < 6 ? 0.7f : 0.5f;
+            float d = (float)days * mul;
+            days = (int16_t)(d == 0 ? 1.0f : d);
+        }
+        if (bad) {
+            A.worstState[i] = (int8_t)st;
+            A.worstTs[i] = ts;
+            A.worstEndTs[i] = ts + (uint32_t)((int)days * 60 * 24) / dt;
+        } else if (old != st) {
+            if (wasInf && !T.stInfected[st]) {
+                A.recTs[i] = ts;
+                A.flags[i] = (uint8_t)(A.flags[i] & ~F_DIAGNOSED);
+            }
+        }
+    }
+    A.days[i] = days;
+}
+
+// ------------------------------------------------------------ immunity
+struct ImmArgs {
+    Agents A;
+    Tables T;
+    uint32_t ts, dt;
+    float eff[5][6][2];  // inf, 2 dose, 3 dose, inf+2, inf+3; rows (comp + 3*different); (waning, level)
+    int sim[MAXV];
+    float simMul[MAXV];
+};
+
+__device__ __forceinline__ float fmin_ref(float a, float b) { return a < b ? a : b; }
+
+// Immunization::update, susceptibility of state-0 agents (immunization.h:547-695)
+__global__ void k_immunization(ImmArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const Agents& A = a.A;
+    if (i >= A.n) return;
+    if (A.state[i] != 0) return;
+    const uint32_t nVar = a.T.nVar;
+    const uint32_t ts = a.ts == 0 ? 1u : a.ts;
+    const uint32_t week = 7u * 24u * 60u / a.dt;
+    float susc[MAXV];
+    for (uint32_t v = 0; v < nVar; ++v) susc[v] = 1.0f;
+    const uint32_t wImm = (ts - A.immTs[i]) / week;
+    const uint32_t wInf = (ts - A.infTs[i]) / week;
+    const uint8_t infC = A.infCount[i], immC = A.immCount[i];
+    const uint32_t prior = A.statVar[i];
+    float scal = A.scal[i];
+    auto lin = [&](int e, int row, uint32_t weeks) -> float {
+        return -1.0f * a.eff[e][row][0] * (float)weeks + a.eff[e][row][1];
+    };
+    auto mix = [&](int e, int comp, int similar, uint32_t v, uint32_t weeks) -> float {
+        int d = (1 - similar) * 3 + comp, s = similar * 3 + comp;
+        return -1.0f * (a.eff[e][d][0] + (a.eff[e][s][0] - a.eff[e][d][0]) * a.simMul[v]) * (float)weeks + a.eff[e][d][1] +
+               (a.eff[e][s][1] - a.eff[e][d][1]) * a.simMul[v];
+    };
+    auto setSlot = [&](uint32_t v, float x) { A.slot6[(size_t)v * A.n + i] = scal * x; };
+    if (infC > 0 && (immC == 0 || (immC == 1 && wImm < 4))) {
+        for (uint32_t v = 0; v < nVar; ++v) {
+            int similar = 0;
+            for (uint32_t j = 0; j < nVar; ++j) similar = similar || ((prior & (1u << j)) && a.sim[v] == a.sim[j]);
+            int d = (1 - similar) * 3;
+            float vsInf = (float)((double)(-1.0f * a.eff[0][d][0] * (float)wInf) +
+                                  (((1u << v) & prior) ? 0.95 : (double)a.eff[0][d][1]));
+            bool mixIt = similar && !((1u << v) & prior) && a.simMul[v] > 0.0f;
+            if (mixIt) vsInf = mix(0, 0, similar, v, wInf);
+            susc[v] = 1.0f - vsInf;
+            float vsSympt = mixIt ? mix(0, 1, similar, v, wInf) : lin(0, d + 1, wInf);
+            float vsHosp = mixIt ? mix(0, 2, similar, v, wInf) : lin(0, d + 2, wInf);
+            setSlot(v, fmin_ref(1.0f, (1.0f - vsHosp) / (1.0f - vsSympt)));  // slots 2, 4, 5 share index 6; the last write wins
+        }
+    } else if (infC == 0 && immC > 0) {
+        for (uint32_t v = 0; v < nVar; ++v) {
+            int similar = a.sim[v] == 0;
+            int d = (1 - similar) * 3;
+            if (immC == 1 && wImm == 0) continue;
+            if (immC == 1 && wImm < 4) {
+                float vsInf = (-1.0f * a.eff[1][d][0] * (float)wImm + a.eff[1][d][1]) / 2.0f;
+                susc[v] = 1.0f - vsInf;
+                float vsSympt = (-1.0f * a.eff[1][d + 1][0] * (float)wImm + a.eff[1][d + 1][1]) / 2.0f;
+                float vsHosp = (-1.0f * a.eff[1][d + 2][0] * (float)wImm + a.eff[1][d + 2][1]) / 2.0f;
+                setSlot(v, (1.0f - vsHosp) / (1.0f - vsSympt));
+            } else {
+                int e = immC == 1 ? 1 : 2;
+                bool mixIt = similar && a.simMul[v] > 0.0f;
+                float vsInf = mixIt ? mix(e, 0, similar, v, wImm) : lin(e, d + 0, wImm);
+                susc[v] = 1.0f - vsInf;
+                float vsSympt = mixIt ? mix(e, 1, similar, v, wImm) : lin(e, d + 1, wImm);
+                float vsHosp = mixIt ? mix(e, 2, similar, v, wImm) : lin(e, d + 2, wImm);
+                setSlot(v, (1.0f - vsHosp) / (1.0f - vsSympt));
+            }
+        }
+    } else if (infC > 0 && immC > 0) {
+        uint32_t weeks = wInf < wImm ? wInf : wImm;
+        int e = immC == 1 ? 3 : 4;
+        for (uint32_t v = 0; v < nVar; ++v) {
+            int similar = 0;
+            for (uint32_t j = 0; j < nVar; ++j) similar = similar || ((prior & (1u << j)) && a.sim[v] == a.sim[j]);
+            int d = (1 - similar) * 3;
+            bool mixIt = similar && !((1u << v) & prior) && a.simMul[v] > 0.0f;
+            float vsInf = (float)((double)(-1.0f * a.eff[e][d][0] * (float)weeks) +
+                                  (((1u << v) & prior) ? 0.95 : (double)a.eff[e][d][1]));
+            if (mixIt) vsInf = mix(e, 0, similar, v, weeks);
+            susc[v] = 1.0f - vsInf;
+            float vsSympt = mixIt ? mix(e, 1, similar, v, weeks) : lin(e, d + 1, weeks);
+            float vsHosp = mixIt ? mix(e, 2, similar, v, weeks) : lin(e, d + 2, weeks);
+            setSlot(v, fmin_ref(1.0f, (1.0f - vsHosp) / (1.0f - vsSympt)));
+        }
+    }
+    for (uint32_t v = 0; v < nVar; ++v) A.susc[(size_t)v * A.n + i] = fmin_ref(1.0f, susc[v]);
+}
+
+// ------------------------------------------------------------ non-COVID hospitalisation
+__constant__ double c_randomHospProbs[84] = {
+    0.00017785,  0.000118567, 0.000614453, 0.003174864, 0.001016573, 0.0011019,   0.000926381, 9.52062E-05,
+    6.34708E-05, 0.000328928, 0.000986754, 0.00044981,  0.000623826, 0.000855563, 0.006333678, 0.004222452,
+    0.008444904, 0.030227275, 0.036202732, 0.03924143,  0.032990764, 0.003390532, 0.002260355, 0.00452071,
+    0.009394699, 0.016018856, 0.022216016, 0.030468759, 0.000364207, 0.000242805, 0.001240249, 0.006390329,
+    0.002081773, 0.002256508, 0.001897075, 9.97603E-05, 6.65068E-05, 0.000335,    0.000999373, 0.000471326,
+    0.000653666, 0.000896488, 0.002751133, 0.001834089, 0.003668178, 0.013129695, 0.015725228, 0.017045135,
+    0.014330059, 0.001472731, 0.000981821, 0.001963642, 0.004080736, 0.006958043, 0.009649877, 0.013234587,
+    0.00036181,  0.000241207, 0.001237053, 0.006378891, 0.002068074, 0.002241659, 0.001884591, 9.84773E-05,
+    6.56515E-05, 0.000333289, 0.000995818, 0.000465264, 0.000645259, 0.000884959, 0.012211473, 0.008140982,
+    0.016281963, 0.058278861, 0.069799674, 0.075658351, 0.063606928, 0.006537022, 0.004358015, 0.008716029,
+    0.018113189, 0.030884712, 0.042832973, 0.058744446};
+
+// Simulation::otherDisease per-agent part (simulation.h:206-422)
+__global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t replica, uint32_t ts, uint32_t dt,
+                                float probabilityMul) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    uint8_t st = A.state[i];
+    if (T.stWB[st] == 4) return;
+    Stream rs(seed, i, ts, PH_OTHER_DISEASE, replica);
+    uint8_t age = A.age[i];
+    uint32_t ag = age < 5 ? 0 : age < 15 ? 1 : age < 30 ? 2 : age < 60 ? 3 : age < 70 ? 4 : age < 80 ? 5 : 6;
+    uint32_t notSex = A.sex[i] ? 0u : 1u;
+    uint8_t pc = A.precond[i];
+    uint32_t type = pc == 2 ? 1 : (pc == 4 ? 2 : 0);
+    if (ts > 0 && A.hospUntil[i] == ts) {
+        if (rs.real(1.0) < c_randomHospProbs[type * 4 * 7 + 2 * 7 + notSex * 7 + ag]) {
+            // die(false) (dynamicPPState.cpp:313-325)
+            uint8_t d = (uint8_t)T.nonCOVIDDead;
+            A.state[i] = d;
+            A.days[i] = -1;
+            update_meta(A, T, i, d);
+            A.worstState[i] = (int8_t)d;
+            A.worstTs[i] = ts;
+            return;
+        }
+    }
+    uint8_t w = T.stWB[st];
+    if (w == 4 || w == 3 || ts < A.hospUntil[i]) return;
+    (void)rs.real(1.0);  // sudden death: disabled in the reference, draw still made
+    const int multDay[4] = {99, 99 + 365, 99 + 2 * 365, 99 + 3 * 365};
+    const double mult[4] = {0.82, 0.67, 0.9, 0.9};
+    int day = (int)(ts / (24u * 60u / dt));
+    int d = 0;
+    while (d < 3 && day > multDay[d]) d++;
+    double probability = c_randomHospProbs[type * 4 * 7 + notSex * 7 + ag] *
+                         (probabilityMul > 0.0f ? (double)probabilityMul : mult[d]);
+    if (rs.real(1.0) < probability) {
+        const double avgLengths[3] = {5.55, 2.78, 5.24};
+        unsigned avg = (unsigned)avgLengths[type];
+        double p = 1.0 / (double)avg;
+        unsigned length = (unsigned)geometric(p, rs);
+        if (length == 0) length = 1;
+        A.hospTs[i] = ts;
+        A.hospUntil[i] = ts + length * 24u * 60u / dt;
+    }
+}
+
+__global__ void k_count_covid_hospital(Agents A, Locs L, Tables T, uint32_t ts, uint32_t dueWith,
+                                       unsigned long long* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool c = false;
+    if (i < A.n) {
+        uint8_t st = A.state[i];
+        if (dueWith == 0) c = st >= 6 && st <= 8;
+        else
+            c = T.stInfected[st] && L.type[A.location[i]] == T.lt.hospital &&
+                !(T.stWB[st] < 3 && ts > 0 && A.hospUntil[i] == ts);
+    }
+    unsigned long long m = __ballot(c);
+    if ((threadIdx.x & 63u) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
+// ------------------------------------------------------------ statistics
+enum StatSlot {
+    S_HOSP = 0, S_STAYED, S_Q, S_QT, S_NQ, S_ALLINF, S_NI, S_INF, S_REINF, S_BSTR, S_HCI, S_HCE, S_INFH, S_VNI,
+    S_VARINF = 14,            // + v
+    S_INFV = S_VARINF + MAXV, // + v
+    S_IMM = S_INFV + MAXV,    // + v (2^-24 fixed point)
+    S_STATE = S_IMM + MAXV,   // + state
+    S_COUNT = S_STATE + 128
+};
+
+__global__ void __launch_bounds__(256) k_stats(Agents A, Locs L, Tables T, uint32_t ts, uint32_t dt,
+                                               unsigned long long* out) {
+    __shared__ unsigned long long sh[S_COUNT];
+    for (uint32_t k = threadIdx.x; k < S_COUNT; k += blockDim.x) sh[k] = 0;
+    __syncthreads();
+    const uint32_t day = 24u * 60u / dt;
+    const uint32_t nVar = T.nVar;
+    uint32_t stride = gridDim.x * blockDim.x;
+    unsigned int c[14] = {0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+        uint8_t st = A.state[i];
+        uint8_t w = T.stWB[st];
+        uint8_t fl = A.flags[i];
+        bool inf = T.stInfected[st] != 0;
+        atomicAdd(&sh[S_STATE + st], 1ull);
+        c[S_HOSP] += (w != 4 && w != 3 && ts < A.hospUntil[i]);
+        c[S_STAYED] += (fl & F_STAYED_HOME) ? 1 : 0;
+        bool isQ = A.quarTs[i] <= ts && A.quarUntil[i] > ts;
+        c[S_Q] += isQ;
+        c[S_QT] += isQ && inf;
+        c[S_NQ] += !isQ && inf;
+        c[S_ALLINF] += inf;
+        uint32_t it = A.infTs[i];
+        bool newInf = it > ts - day && it <= ts;
+        c[S_NI] += newInf;
+        uint8_t ic = A.infCount[i], mc = A.immCount[i];
+        c[S_INF] += ic > 0;
+        c[S_REINF] += ic > 1 ? ic - 1u : 0u;
+        c[S_BSTR] += mc > 1 ? mc - 1u : 0u;
+        if (fl & F_HCW) {
+            c[S_HCI] += inf || (fl & F_QUARANTINED);
+            c[S_HCE] += newInf;
+        }
+        c[S_INFH] += inf && L.type[A.location[i]] == T.lt.hospital;
+        c[S_VNI] += ic == 0 && mc > 0;
+        if (inf) atomicAdd(&sh[S_VARINF + A.variant[i]], 1ull);
+        uint8_t sv = A.statVar[i];
+        for (uint32_t v = 0; v < nVar; ++v) {
+            if ((sv >> v) & 1u) atomicAdd(&sh[S_INFV + v], 1ull);
+            float s = A.susc[(size_t)v * A.n + i];
+            long long q = (long long)((double)(1.0f - s) * 16777216.0);
+            if (q) atomicAdd(&sh[S_IMM + v], (unsigned long long)q);
+        }
+    }
+    for (int k = 0; k < 14; ++k)
+        if (c[k]) atomicAdd(&sh[k], (unsigned long long)c[k]);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < S_COUNT; k += blockDim.x)
+        if (sh[k]) atomicAdd(&out[k], sh[k]);
+}
+
+// ------------------------------------------------------------ planning
+struct PlanArgs {
+    Agents A;
+    Locs L;
+    Tables T;
+    uint64_t seed;
+    uint32_t replica, ts, dt, curfewEnd, schoolAge;
+};
+
+// resets of planLocations (movementPolicies.h:1446-1451)
+__global__ void k_plan_reset(PlanArgs a) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = gid; i < a.A.n; i += stride) {
+        // stepsUntilMove := curfewEnd at this midnight step
+        a.A.wake[i] = a.ts + a.curfewEnd;
+        a.A.flags[i] = (uint8_t)((a.A.flags[i] | F_STAYED_HOME) & ~F_NO_WORK);
+    }
+    for (uint32_t l = gid; l < a.L.n; l += stride) a.L.noWork[l] = 0u;
+}
+
+// checkSchoolQuarantine (movementPolicies.h:1285-1310), one thread per school
+__global__ void k_plan_school(PlanArgs a) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.L.nSchools) return;
+    uint32_t* q = a.L.quarUntil;
+    uint32_t school = a.L.schools[s];
+    if (q[school] > a.ts) return;
+    uint32_t counter = 0, mx = 0;
+    for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) {
+        uint32_t cl = a.L.classrooms[c];
+        if (q[cl] > a.ts) {
+            counter++;
+            mx = mx > q[cl] ? mx : q[cl];
+        }
+    }
+    if (counter > 1) {
+        q[school] = mx;
+        for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) q[a.L.classrooms[c]] = mx;
+    }
+}
+
+// checkSchoolWorkQuarantine (movementPolicies.h:1188-1240)
+__global__ void k_plan_schoolwork(PlanArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const Agents& A = a.A;
+    if (i >= A.n) return;
+    uint8_t fl = A.flags[i];
+    if (fl & F_QUARANTINED) return;
+    if (A.diagTs[i] > 0 && !a.T.stInfected[A.state[i]]) return;
+    const pansim_location_types& lt = a.T.lt;
+    Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
+    uint32_t wl = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
+    uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
+    uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
+    const uint32_t* q = a.L.quarUntil;
+    uint32_t until = 0;
+    if (sl != NOLOC && q[sl] > a.ts) until = q[sl];
+    else if (cl != NOLOC && q[cl] > a.ts) until = q[cl];
+    else if (wl != NOLOC && q[wl] > a.ts) until = q[wl];
+    if (until > 0) {
+        A.flags[i] = (uint8_t)(fl | F_QUARANTINED);
+        A.quarTs[i] = a.ts;
+        A.quarUntil[i] = until;
+        A.daysInQ[i] += (until - a.ts) / (24u * 60u / a.dt);
+    }
+}
+
+// quarantine -> closedUntil conversion (movementPolicies.h:1544-1557)
+__global__ void k_plan_close(PlanArgs a) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    const pansim_location_types& lt = a.T.lt;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < a.L.n; l += stride) {
+        uint32_t q = a.L.quarUntil[l];
+        uint32_t t = a.L.type[l];
+        if (q > a.ts && (t == lt.school || t == lt.classroom)) {
+            a.L.closedUntil[l] = q;
+            a.L.quarUntil[l] = 0;
+        } else if (q > a.ts && t == lt.work) {
+            a.L.quarUntil[l] = 0;
+        }
+    }
+}
+
+constexpr uint32_t NOWORK_FLAGGED = 0xFFFFFFFFu;
+
+// checkUnderageAtHome (movementPolicies.h:1056-1094)
+__global__ void k_plan_underage(PlanArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const Agents& A = a.A;
+    if (i >= A.n) return;
+    if (A.age[i] > 14) return;
+    const pansim_location_types& lt = a.T.lt;
+    bool flag = false;
+    if (A.flags[i] & F_QUARANTINED) flag = true;
+    else {
+        Stream rs(a.seed, i, a.ts, PH_PLAN_UNDERAGE, a.replica);
+        uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
+        uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
+        bool tooOld = A.age[i] >= a.schoolAge;
+        flag = tooOld || (sl != NOLOC && (a.L.open[sl] == 0 || a.L.closedUntil[sl] > a.ts)) ||
+               (cl != NOLOC && (a.L.open[cl] == 0 || a.L.closedUntil[cl] > a.ts));
+    }
+    if (flag) {
+        uint32_t h = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
+        if (h != NOLOC) a.L.noWork[h] = NOWORK_FLAGGED;
+    }
+}
+
+// setNoWorkToday (movementPolicies.h:1137-1158): the lowest eligible agent of
+// a flagged home claims it (claim pass, then resolve pass)
+__global__ void k_plan_nowork_claim(PlanArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const Agents& A = a.A;
+    if (i >= A.n) return;
+    if (!(A.age[i] > 26 && A.age[i] < 65)) return;
+    uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
+    // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent + 1)
+    if (h != NOLOC && a.L.noWork[h] != 0u) atomicMin(&a.L.noWork[h], i + 1u);
+}
+
+__global__ void k_plan_nowork_resolve(PlanArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const Agents& A = a.A;
+    if (i >= A.n) return;
+    if (!(A.age[i] > 26 && A.age[i] < 65)) return;
+    uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
+    if (h != NOLOC && a.L.noWork[h] == i + 1u) A.flags[i] = (uint8_t)(A.flags[i] | F_NO_WORK);
+}
+
+// ------------------------------------------------------------ closures
+__global__ void k_set_open(Locs L, const uint32_t* types, uint32_t n, uint8_t open) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
+        if (L.essential[l] == 1) continue;
+        for (uint32_t k = 0; k < n; ++k)
+            if (L.type[l] == types[k]) L.open[l] = open;
+    }
+}
+
+__global__ void k_scale_inf(Locs L, uint32_t homeType, double factor, uint32_t divide) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride)
+        if (L.type[l] != homeType) L.inf[l] = divide ? (float)((double)L.inf[l] / factor) : (float)((double)L.inf[l] * factor);
+}
+
+// ExposeToMutation (closurePolicies.h:356-390)
+__global__ void k_expose_mutation(Agents A, Tables T, uint64_t seed, uint32_t replica, uint32_t ts, uint32_t v,
+                                  double fraction) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    float s = A.susc[(size_t)v * A.n + i];
+    if (!(s > 0.0f)) return;
+    Stream rs(seed, i, ts, PH_MUTATION, replica);
+    if (rs.unit() < fraction * (double)s) {
+        uint8_t fi = (uint8_t)T.firstInfected;
+        A.state[i] = fi;
+        A.variant[i] = (uint8_t)v;
+        A.days[i] = -2;
+        update_meta(A, T, i, fi);
+        A.infTs[i] = ts;
+        A.infLoc[i] = A.location[i];
+        A.infCount[i] = (uint8_t)(A.infCount[i] + 1);
+        A.worstState[i] = (int8_t)fi;
+        A.worstTs[i] = ts;
+        A.statVar[i] = (uint8_t)(A.statVar[i] | (1u << v));
+    }
+}
+
+// ------------------------------------------------------------ test helpers
+__global__ void k_expf(const float* in, float* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ps_expf(in[i]);
+}
+__global__ void k_log(const double* in, double* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ps_log(in[i]);
+}
+__global__ void k_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u32x4 c;
+    for (int j = 0; j < 4; ++j) c.v[j] = ctr[4 * i + j];
+    u32x4 r = philox4x32_10(c, k0, k1);
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = r.v[j];
+}
+
+}  // namespace dev
+}  // namespace ps

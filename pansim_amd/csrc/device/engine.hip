@@ -1,0 +1,749 @@
+// C ABI of the gfx950 agent pipeline (include/pansim_hip.h): context with
+// the device arrays of one population, its stream, and the launch sequences
+// of every per-step and per-midnight routine.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "day_kernels.h"
+#include "engine_types.h"
+#include "step_kernels.h"
+
+using namespace ps::dev;
+
+struct pansim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    pansim_dims dims{};
+    uint64_t seed = 0;
+    uint32_t replica = 0;
+    Agents A{};
+    Locs L{};
+    Tables T{};
+    StepScratch S{};
+    std::vector<void*> allocations;
+    unsigned long long* counters = nullptr;  // device scratch for reductions (S_COUNT + 16)
+    unsigned long long* hostCounters = nullptr;
+    uint32_t hcwCount = 0;
+    int numCUs = 256;
+    bool timing = false;
+    // per-call HIP event pairs on the context stream while timing is on
+    std::vector<hipEvent_t> moveEv, infectEv;
+    size_t moveN = 0, infectN = 0;
+    unsigned long long* moveCounters = nullptr;  // evaluated movers, relocated agents
+    // schedule storage (re-uploadable)
+    uint32_t* evOffDev = nullptr;
+    pansim_event* evDev = nullptr;
+    uint32_t evOffCap = 0, evCap = 0;
+    uint32_t nEvents = 0;
+};
+
+namespace {
+
+std::string g_create_error;
+
+int fail(pansim_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIP_OR_FAIL(ctx, call)                                                                        \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(ctx, PANSIM_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+template <class T>
+int dalloc(pansim_ctx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, PANSIM_E_DEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    c->allocations.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+template <class T>
+int upload(pansim_ctx* c, T** p, const T* src, size_t count) {
+    int r = dalloc(c, p, count);
+    if (r) return r;
+    if (src && count) HIP_OR_FAIL(c, hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+inline unsigned blocks_for(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+int check_launch(pansim_ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, PANSIM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+unsigned persistent_blocks(pansim_ctx* c) { return (unsigned)c->numCUs * 8u; }
+
+}  // namespace
+
+extern "C" {
+
+const char* pansim_create_error(void) { return g_create_error.c_str(); }
+const char* pansim_last_error(const pansim_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t, const pansim_locations* l,
+                  const pansim_agents* a, uint64_t seed, uint32_t replica, uint32_t time_step, uint32_t quarantine_policy,
+                  uint32_t quarantine_length) {
+    if (!out || !d || !t || !l || !a) {
+        g_create_error = "pansim_create: null argument";
+        return PANSIM_E_ARG;
+    }
+    *out = nullptr;
+    if (d->n_agents == 0 || d->n_locations == 0 || d->n_variants == 0 || d->n_variants > 7 || d->n_states == 0 ||
+        d->n_states > 127 || time_step == 0 || 1440 % time_step != 0) {
+        g_create_error = "pansim_create: invalid dimensions";
+        return PANSIM_E_ARG;
+    }
+    // the fused infection pass assumes susceptible states carry no infectiousness
+    for (uint32_t s = 0; s < d->n_states; ++s)
+        if (t->state_susceptible[s] && t->state_infectious[s] != 0.0f) {
+            g_create_error = "pansim_create: a susceptible state with non-zero infectiousness is not supported";
+            return PANSIM_E_UNSUPPORTED;
+        }
+    if (t->state_infectious[t->first_infected] != 0.0f) {
+        g_create_error = "pansim_create: non-zero infectiousness of the first infected state is not supported";
+        return PANSIM_E_UNSUPPORTED;
+    }
+    pansim_ctx* c = new pansim_ctx();
+    c->dims = *d;
+    c->seed = seed;
+    c->replica = replica;
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        pansim_destroy(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        c->err = "no HIP device";
+        return bail(PANSIM_E_DEVICE);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->numCUs = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(PANSIM_E_DEVICE);
+    }
+    const uint32_t N = d->n_agents, NL = d->n_locations, V = d->n_variants, NS = d->n_states;
+    const uint32_t NP = d->n_progressions;
+    int r = 0;
+#define TRY(x)                  \
+    do {                        \
+        if ((r = (x)) != 0) return bail(r); \
+    } while (0)
+    // ---- tables
+    Tables& T = c->T;
+    T.nStates = NS;
+    T.nVar = V;
+    T.nProg = NP;
+    T.firstInfected = t->first_infected;
+    T.nonCOVIDDead = t->non_covid_dead;
+    T.deadState = t->dead_state;
+    T.lt = t->lt;
+    TRY(upload(c, (float**)&T.stInf, t->state_infectious, NS));
+    TRY(upload(c, (uint8_t**)&T.stWB, t->state_wb, NS));
+    TRY(upload(c, (uint8_t**)&T.stSusc, t->state_susceptible, NS));
+    TRY(upload(c, (uint8_t**)&T.stInfected, t->state_infected, NS));
+    TRY(upload(c, (float**)&T.accPCR, t->state_accuracy_pcr, NS));
+    TRY(upload(c, (float**)&T.accAnt, t->state_accuracy_antigen, NS));
+    TRY(upload(c, (float**)&T.varMul, t->variant_multiplier, V));
+    TRY(upload(c, (float**)&T.progScaling, t->progression_scaling, V));
+    TRY(upload(c, (float**)&T.deathScaling, t->death_scaling, V));
+    TRY(upload(c, (uint32_t**)&T.edgeOff, t->edge_off, (size_t)NP * NS + 1));
+    TRY(upload(c, (uint32_t**)&T.badCount, t->bad_count, (size_t)NP * NS));
+    TRY(upload(c, (uint32_t**)&T.edgeTo, t->edge_to, d->n_matrix_edges));
+    TRY(upload(c, (float**)&T.edgeChance, t->edge_chance, d->n_matrix_edges));
+    TRY(upload(c, (float**)&T.lenAvg, t->length_avg, (size_t)NP * NS * 7));
+    TRY(upload(c, (float**)&T.lenMax, t->length_max, (size_t)NP * NS * 7));
+    TRY(upload(c, (double**)&T.lenP, t->length_p, (size_t)NP * NS * 7));
+    c->evOffCap = d->n_agent_types * 28 + 1;
+    c->evCap = d->n_events + 4096;
+    TRY(dalloc(c, &c->evOffDev, c->evOffCap));
+    TRY(dalloc(c, &c->evDev, c->evCap));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->evOffDev, t->event_offset, c->evOffCap * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    if (d->n_events)
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->evDev, t->events, d->n_events * sizeof(pansim_event), hipMemcpyHostToDevice, c->stream));
+    T.evOff = c->evOffDev;
+    T.ev = c->evDev;
+    c->nEvents = d->n_events;
+    // ---- locations
+    Locs& L = c->L;
+    L.n = NL;
+    L.cemetery = l->cemetery;
+    L.nSchools = d->n_schools;
+    {
+        std::vector<uint8_t> ty(NL);
+        for (uint32_t i = 0; i < NL; ++i) ty[i] = (uint8_t)l->type[i];
+        TRY(upload(c, &L.type, ty.data(), NL));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    }
+    TRY(upload(c, &L.open, l->open, NL));
+    TRY(upload(c, &L.essential, l->essential, NL));
+    TRY(dalloc(c, &L.flag, NL));
+    TRY(dalloc(c, &L.closedUntil, NL));
+    TRY(dalloc(c, &L.quarUntil, NL));
+    TRY(dalloc(c, &L.noWork, NL));
+    TRY(dalloc(c, &L.count, NL));
+    TRY(upload(c, &L.inf, l->infectiousness, NL));
+    TRY(dalloc(c, &L.sum, (size_t)V * NL));
+    TRY(dalloc(c, &L.pendKey, NL));
+    TRY(upload(c, (uint32_t**)&L.schools, l->schools, d->n_schools));
+    TRY(upload(c, (uint32_t**)&L.classrooms, l->classrooms, d->n_classrooms));
+    TRY(upload(c, (uint32_t**)&L.classOff, l->classroom_offsets, d->n_schools + 1));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.flag, 0, NL, c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.closedUntil, 0, NL * sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.quarUntil, 0, NL * sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.noWork, 0, NL * sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.pendKey, 0, NL * sizeof(unsigned long long), c->stream));
+    // ---- agents
+    Agents& A = c->A;
+    A.n = N;
+    TRY(upload(c, &A.state, a->state, N));
+    TRY(dalloc(c, &A.variant, N));
+    TRY(upload(c, &A.prog, a->progression, N));
+    TRY(upload(c, &A.type, a->agent_type, N));
+    TRY(upload(c, &A.age, a->age, N));
+    TRY(upload(c, &A.sex, a->sex, N));
+    TRY(upload(c, &A.precond, a->precond, N));
+    {
+        std::vector<uint8_t> fl(N);
+        uint32_t hcw = 0;
+        for (uint32_t i = 0; i < N; ++i) {
+            fl[i] = (uint8_t)((a->diagnosed[i] ? F_DIAGNOSED : 0) | (a->healthcare_worker[i] ? F_HCW : 0));
+            hcw += a->healthcare_worker[i] ? 1 : 0;
+        }
+        c->hcwCount = hcw;
+        TRY(upload(c, &A.flags, fl.data(), N));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    }
+    TRY(dalloc(c, &A.days, N));
+    TRY(upload(c, &A.location, a->location, N));
+    TRY(dalloc(c, &A.wake, N));
+    TRY(dalloc(c, &A.lastTest, N));
+    TRY(upload(c, &A.scal, a->scaling_age_sex, N));
+    TRY(dalloc(c, &A.susc, (size_t)V * N));
+    TRY(dalloc(c, &A.slot6, (size_t)V * N));
+    uint32_t** u32s[] = {&A.infTs, &A.infLoc, &A.worstTs, &A.worstEndTs, &A.diagTs, &A.quarTs,
+                         &A.quarUntil, &A.daysInQ, &A.hospTs, &A.hospUntil, &A.immTs, &A.recTs};
+    for (auto p : u32s) TRY(dalloc(c, p, N));
+    TRY(dalloc(c, &A.immCount, N));
+    TRY(dalloc(c, &A.statVar, N));
+    TRY(dalloc(c, &A.infCount, N));
+    TRY(dalloc(c, &A.worstState, N));
+    TRY(upload(c, (uint32_t**)&A.locOffset, a->loc_offset, (size_t)N + 1));
+    TRY(upload(c, (uint32_t**)&A.possLoc, a->possible_locations, d->n_csr));
+    TRY(upload(c, (uint8_t**)&A.possType, a->possible_types, d->n_csr));
+    // ---- scratch
+    TRY(dalloc(c, &c->S.queue, N));
+    TRY(dalloc(c, &c->S.queueCount, 1));
+    c->S.pendCap = 3u * N + 16u;
+    TRY(dalloc(c, &c->S.pendList, c->S.pendCap));
+    TRY(dalloc(c, &c->S.pendCount, 1));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->S.queueCount, 0, sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->S.pendCount, 0, sizeof(uint32_t), c->stream));
+    TRY(dalloc(c, &c->counters, S_COUNT + 16));
+    if (hipHostMalloc((void**)&c->hostCounters, (S_COUNT + 16) * sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipSuccess) {
+        c->err = "hipHostMalloc failed";
+        return bail(PANSIM_E_DEVICE);
+    }
+    TRY(dalloc(c, &c->moveCounters, 2));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->moveCounters, 0, 2 * sizeof(unsigned long long), c->stream));
+    // ---- initial agent state
+    uint32_t qlenSteps = quarantine_length * 24u * 60u / time_step;
+    hipLaunchKernelGGL(k_init_agents, dim3(blocks_for(N)), dim3(256), 0, c->stream, c->A, c->T, seed, replica,
+                       quarantine_policy, qlenSteps, quarantine_length);
+    if ((r = check_launch(c, "k_init_agents")) != 0) return bail(r);
+    hipLaunchKernelGGL(k_zero_locations, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.nVar);
+    hipLaunchKernelGGL(k_scatter_locations, dim3(blocks_for(N)), dim3(256), 0, c->stream, c->A, c->L, c->T);
+    if ((r = check_launch(c, "initial occupancy")) != 0) return bail(r);
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+#undef TRY
+    *out = c;
+    return 0;
+}
+
+void pansim_destroy(pansim_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocations) hipFree(p);
+    if (c->hostCounters) hipHostFree(c->hostCounters);
+    for (auto e : c->moveEv) hipEventDestroy(e);
+    for (auto e : c->infectEv) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pansim_sync(pansim_ctx* c) {
+    if (!c) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pansim_enable_timing(pansim_ctx* c, uint32_t on) {
+    if (!c) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    c->timing = on != 0;
+    c->moveN = c->infectN = 0;
+    return 0;
+}
+
+static int record_event(pansim_ctx* c, std::vector<hipEvent_t>& v, size_t idx) {
+    while (v.size() <= idx) {
+        hipEvent_t e;
+        HIP_OR_FAIL(c, hipEventCreate(&e));
+        v.push_back(e);
+    }
+    HIP_OR_FAIL(c, hipEventRecord(v[idx], c->stream));
+    return 0;
+}
+
+int pansim_timing_summary(pansim_ctx* c, uint32_t* n_move, double* move_ms, uint32_t* n_infect, double* infect_ms) {
+    if (!c) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    double m = 0.0, f = 0.0;
+    for (size_t k = 0; k + 1 < 2 * c->moveN + 1 && 2 * k + 1 < c->moveEv.size() && k < c->moveN; ++k) {
+        float x = 0.f;
+        HIP_OR_FAIL(c, hipEventElapsedTime(&x, c->moveEv[2 * k], c->moveEv[2 * k + 1]));
+        m += x;
+    }
+    for (size_t k = 0; k < c->infectN && 2 * k + 1 < c->infectEv.size(); ++k) {
+        float x = 0.f;
+        HIP_OR_FAIL(c, hipEventElapsedTime(&x, c->infectEv[2 * k], c->infectEv[2 * k + 1]));
+        f += x;
+    }
+    if (n_move) *n_move = (uint32_t)c->moveN;
+    if (move_ms) *move_ms = m;
+    if (n_infect) *n_infect = (uint32_t)c->infectN;
+    if (infect_ms) *infect_ms = f;
+    return 0;
+}
+
+int pansim_last_kernel_ms(pansim_ctx* c, float* move_ms, float* infect_ms) {
+    if (!c) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    float m = 0.f, f = 0.f;
+    if (c->moveN) HIP_OR_FAIL(c, hipEventElapsedTime(&m, c->moveEv[2 * (c->moveN - 1)], c->moveEv[2 * (c->moveN - 1) + 1]));
+    if (c->infectN)
+        HIP_OR_FAIL(c, hipEventElapsedTime(&f, c->infectEv[2 * (c->infectN - 1)], c->infectEv[2 * (c->infectN - 1) + 1]));
+    if (move_ms) *move_ms = m;
+    if (infect_ms) *infect_ms = f;
+    return 0;
+}
+
+int pansim_movement_counters(pansim_ctx* c, uint64_t* evaluated, uint64_t* relocated, uint32_t reset) {
+    if (!c) return PANSIM_E_ARG;
+    unsigned long long h[2];
+    HIP_OR_FAIL(c, hipMemcpyAsync(h, c->moveCounters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    if (evaluated) *evaluated = h[0];
+    if (relocated) *relocated = h[1];
+    if (reset) HIP_OR_FAIL(c, hipMemsetAsync(c->moveCounters, 0, sizeof(h), c->stream));
+    return 0;
+}
+
+int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
+    if (!c || !p) return PANSIM_E_ARG;
+    MoveArgs m;
+    m.A = c->A;
+    m.L = c->L;
+    m.T = c->T;
+    m.S = c->S;
+    m.p = *p;
+    m.seed = c->seed;
+    m.replica = c->replica;
+    const uint32_t minutes = (p->timestamp % (1440u / p->time_step)) * p->time_step;
+    m.curfewStep = (p->enable_curfew && p->curfew_begin == minutes / p->time_step) ? 1u : 0u;
+    m.nEvents = c->nEvents;
+    m.counters = c->moveCounters;
+    if (c->timing && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+    hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(64), 0, c->stream, c->S.queueCount, c->S.pendCount);
+    hipLaunchKernelGGL(k_move_scan, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A.wake, c->A.n, p->timestamp,
+                       m.curfewStep, c->S.queue, c->S.queueCount);
+    hipLaunchKernelGGL(k_move, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, m);
+    hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, c->moveCounters);
+    if (c->timing) {
+        if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
+        c->moveN++;
+    }
+    return check_launch(c, "movement");
+}
+
+int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, float k, float season, uint32_t dt) {
+    if (!c) return PANSIM_E_ARG;
+    InfectArgs a;
+    a.A = c->A;
+    a.L = c->L;
+    a.T = c->T;
+    a.S = c->S;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.timestamp = timestamp;
+    a.mask = mask & ((1u << c->T.nVar) - 1u);
+    a.k = k;
+    a.season = season;
+    a.dt = (float)dt;
+    unsigned grid = blocks_for(c->A.n);
+    if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
+    hipLaunchKernelGGL(k_infect, dim3(grid), dim3(256), 0, c->stream, a);
+    if (c->timing) {
+        if (record_event(c, c->infectEv, 2 * c->infectN + 1)) return PANSIM_E_DEVICE;
+        c->infectN++;
+    }
+    return check_launch(c, "infection");
+}
+
+static int rebuild_locations(pansim_ctx* c) {
+    hipLaunchKernelGGL(k_zero_locations, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.nVar);
+    hipLaunchKernelGGL(k_scatter_locations, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->L, c->T);
+    return check_launch(c, "rebuild_locations");
+}
+
+int pansim_rebuild_locations(pansim_ctx* c) {
+    if (!c) return PANSIM_E_ARG;
+    return rebuild_locations(c);
+}
+
+int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
+    if (!c || !p) return PANSIM_E_ARG;
+    PlanArgs a;
+    a.A = c->A;
+    a.L = c->L;
+    a.T = c->T;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.ts = p->timestamp;
+    a.dt = p->time_step;
+    a.curfewEnd = p->curfew_end;
+    a.schoolAge = p->school_age_restriction;
+    const unsigned gN = blocks_for(c->A.n);
+    hipLaunchKernelGGL(k_plan_reset, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a);
+    if (p->quarantine_policy > 3 && c->L.nSchools)
+        hipLaunchKernelGGL(k_plan_school, dim3(blocks_for(c->L.nSchools)), dim3(256), 0, c->stream, a);
+    if (p->quarantine_policy > 2) {
+        hipLaunchKernelGGL(k_plan_schoolwork, dim3(gN), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_plan_close, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a);
+    }
+    hipLaunchKernelGGL(k_plan_underage, dim3(gN), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_plan_nowork_claim, dim3(gN), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_plan_nowork_resolve, dim3(gN), dim3(256), 0, c->stream, a);
+    int r = check_launch(c, "plan_locations");
+    if (r) return r;
+    return rebuild_locations(c);
+}
+
+int pansim_testing_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const double probs[6], uint32_t delay, uint32_t qlen,
+                       uint32_t usePCR, uint32_t out[3]) {
+    if (!c || !probs) return PANSIM_E_ARG;
+    TestArgs a;
+    a.A = c->A;
+    a.L = c->L;
+    a.T = c->T;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.ts = ts;
+    a.dt = dt;
+    a.delay = delay;
+    a.qlen = qlen;
+    a.usePCR = usePCR;
+    a.pRandom = probs[0];
+    a.pHome = probs[1];
+    a.pWork = probs[2];
+    a.pSchool = probs[3];
+    a.pHospital = probs[4];
+    a.pNursery = probs[5];
+    a.counters = c->counters;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, 3 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_clear_flags, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L);
+    hipLaunchKernelGGL(k_flag_locations, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_do_testing, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+    int r = check_launch(c, "testing");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    if (out)
+        for (int i = 0; i < 3; ++i) out[i] = (uint32_t)c->hostCounters[i];
+    return 0;
+}
+
+int pansim_progression_day(pansim_ctx* c, uint32_t ts, uint32_t dt) {
+    if (!c) return PANSIM_E_ARG;
+    hipLaunchKernelGGL(k_progression, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
+                       ts, dt);
+    return check_launch(c, "progression");
+}
+
+int pansim_immunization_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const float prot[60], const int32_t sim[7],
+                            const float simMul[7]) {
+    if (!c || !prot || !sim || !simMul) return PANSIM_E_ARG;
+    ImmArgs a;
+    a.A = c->A;
+    a.T = c->T;
+    a.ts = ts;
+    a.dt = dt;
+    // prot = level[10] (inf), waning[10] (inf), level[10] (sympt), waning[10], level[10] (hosp), waning[10]
+    for (int e = 0; e < 5; ++e)
+        for (int s = 0; s < 2; ++s)
+            for (int comp = 0; comp < 3; ++comp) {
+                a.eff[e][s * 3 + comp][0] = prot[comp * 20 + 10 + 2 * e + s];
+                a.eff[e][s * 3 + comp][1] = prot[comp * 20 + 2 * e + s];
+            }
+    for (int v = 0; v < MAXV; ++v) {
+        a.sim[v] = sim[v];
+        a.simMul[v] = simMul[v];
+    }
+    hipLaunchKernelGGL(k_immunization, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+    return check_launch(c, "immunization");
+}
+
+int pansim_other_disease_day(pansim_ctx* c, uint32_t ts, uint32_t dt, float probabilityMul) {
+    if (!c) return PANSIM_E_ARG;
+    hipLaunchKernelGGL(k_other_disease, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed,
+                       c->replica, ts, dt, probabilityMul);
+    return check_launch(c, "other_disease");
+}
+
+int pansim_count_hospitalized_covid(pansim_ctx* c, uint32_t ts, uint32_t dueWith, uint32_t* out) {
+    if (!c || !out) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_count_covid_hospital, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->L, c->T, ts,
+                       dueWith, c->counters);
+    int r = check_launch(c, "count_hospitalized");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    *out = (uint32_t)c->hostCounters[0];
+    return 0;
+}
+
+int pansim_count_variants(pansim_ctx* c, uint32_t* out) {
+    if (!c || !out) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, MAXV * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_count_variants, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->counters);
+    int r = check_launch(c, "count_variants");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, MAXV * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (uint32_t v = 0; v < c->T.nVar; ++v) out[v] = (uint32_t)c->hostCounters[v];
+    return 0;
+}
+
+int pansim_stats_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const uint32_t test[3], uint32_t immunizedToday,
+                     uint32_t* out, size_t cap, size_t* len) {
+    if (!c || !test) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, S_COUNT * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_stats, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->A, c->L, c->T, ts, dt, c->counters);
+    int r = check_launch(c, "stats");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    const unsigned long long* h = c->hostCounters;
+    const uint32_t NS = c->T.nStates, V = c->T.nVar;
+    std::vector<uint32_t> v;
+    for (uint32_t s = 0; s < NS; ++s) v.push_back((uint32_t)h[S_STATE + s]);
+    v.push_back((uint32_t)h[S_HOSP]);
+    v.push_back(test[0]);
+    v.push_back(test[1]);
+    v.push_back(test[2]);
+    v.push_back((uint32_t)h[S_Q]);
+    v.push_back((uint32_t)h[S_QT]);
+    v.push_back((uint32_t)h[S_NQ]);
+    if (V > 1) {
+        for (uint32_t k = 0; k + 1 < V; ++k) {
+            double pct = double(h[S_VARINF + k + 1]) / double(h[S_ALLINF]) * 100.0;
+            v.push_back(pct != pct ? 0u : (uint32_t)(int64_t)pct);
+        }
+    } else {
+        v.push_back(0);
+    }
+    uint32_t d10 = NS > 10 ? v[10] : 0, d11 = NS > 11 ? v[11] : 0;
+    v.push_back((uint32_t)h[S_STAYED] - d10 - d11);
+    v.push_back(immunizedToday);
+    v.push_back((uint32_t)h[S_NI]);
+    v.push_back((uint32_t)h[S_INF]);
+    v.push_back((uint32_t)h[S_REINF]);
+    v.push_back((uint32_t)h[S_BSTR]);
+    for (uint32_t k = 0; k < V; ++k) v.push_back((uint32_t)(h[S_IMM + k] >> 24));
+    uint32_t hcw = c->hcwCount > 0 ? c->hcwCount : 1u;
+    v.push_back(((uint32_t)h[S_HCI] * 100u) / hcw);
+    v.push_back((uint32_t)h[S_HCE]);
+    for (uint32_t k = 0; k < V; ++k) v.push_back((uint32_t)h[S_INFV + k]);
+    v.push_back((uint32_t)h[S_INFH]);
+    v.push_back((uint32_t)h[S_VNI]);
+    if (len) *len = v.size();
+    if (out)
+        for (size_t i = 0; i < cap && i < v.size(); ++i) out[i] = v[i];
+    return 0;
+}
+
+int pansim_set_open_by_type(pansim_ctx* c, const uint32_t* types, uint32_t n, uint32_t open) {
+    if (!c || (n && !types) || n > 16) return PANSIM_E_ARG;
+    uint32_t* dtypes = (uint32_t*)(c->counters + S_COUNT);  // 16 u64 words of scratch
+    HIP_OR_FAIL(c, hipMemcpyAsync(dtypes, types, n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_set_open, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, dtypes, n, (uint8_t)(open ? 1 : 0));
+    int r = check_launch(c, "set_open_by_type");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pansim_scale_infectiousness(pansim_ctx* c, double factor, uint32_t divide) {
+    if (!c) return PANSIM_E_ARG;
+    hipLaunchKernelGGL(k_scale_inf, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.lt.home, factor, divide);
+    return check_launch(c, "scale_infectiousness");
+}
+
+int pansim_expose_mutation(pansim_ctx* c, uint32_t ts, uint32_t v, double fraction) {
+    if (!c || v >= c->T.nVar) return PANSIM_E_ARG;
+    hipLaunchKernelGGL(k_expose_mutation, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
+                       ts, v, fraction);
+    return check_launch(c, "expose_mutation");
+}
+
+int pansim_upload_schedules(pansim_ctx* c, const uint32_t* off, uint32_t nOff, const pansim_event* ev, uint32_t nEv) {
+    if (!c || !off || nOff != c->evOffCap) return PANSIM_E_ARG;
+    if (nEv > c->evCap) return fail(c, PANSIM_E_ARG, "schedule grew beyond its reserved capacity");
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    HIP_OR_FAIL(c, hipMemcpy(c->evOffDev, off, nOff * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (nEv) HIP_OR_FAIL(c, hipMemcpy(c->evDev, ev, nEv * sizeof(pansim_event), hipMemcpyHostToDevice));
+    c->nEvents = nEv;
+    return 0;
+}
+
+int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_t n) {
+    if (!c || !out || v >= c->T.nVar || n < c->L.n) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    HIP_OR_FAIL(c, hipMemcpy(out, c->L.sum + (size_t)v * c->L.n, c->L.n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t* out, size_t n) {
+    if (!c || !name || !out) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    const uint32_t N = c->A.n, NL = c->L.n;
+    std::string f = name;
+    auto get32 = [&](const uint32_t* src, uint32_t cnt) -> int {
+        if (n < cnt) return fail(c, PANSIM_E_ARG, "output buffer too small");
+        HIP_OR_FAIL(c, hipMemcpy(out, src, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return 0;
+    };
+    auto get8 = [&](const void* src, uint32_t cnt, bool isSigned) -> int {
+        if (n < cnt) return fail(c, PANSIM_E_ARG, "output buffer too small");
+        std::vector<uint8_t> tmp(cnt);
+        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), src, cnt, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < cnt; ++i) out[i] = isSigned ? (uint32_t)(int32_t)(int8_t)tmp[i] : tmp[i];
+        return 0;
+    };
+    auto getFlag = [&](uint8_t bit) -> int {
+        int r = get8(c->A.flags, N, false);
+        if (r) return r;
+        for (uint32_t i = 0; i < N; ++i) out[i] = (out[i] & bit) ? 1u : 0u;
+        return 0;
+    };
+    const Agents& A = c->A;
+    if (f == "state") return get8(A.state, N, false);
+    if (f == "variant") return get8(A.variant, N, false);
+    if (f == "days") {
+        if (n < N) return PANSIM_E_ARG;
+        std::vector<int16_t> tmp(N);
+        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), A.days, N * sizeof(int16_t), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < N; ++i) out[i] = (uint32_t)(int32_t)tmp[i];
+        return 0;
+    }
+    if (f == "location") return get32(A.location, N);
+    if (f == "stepsUntilMove") {
+        int r = get32(A.wake, N);
+        if (r) return r;
+        for (uint32_t i = 0; i < N; ++i) out[i] = out[i] == NEVER ? NEVER : out[i] - ts;
+        return 0;
+    }
+    if (f == "diagnosed") return getFlag(F_DIAGNOSED);
+    if (f == "quarantined") return getFlag(F_QUARANTINED);
+    if (f == "stayedHome") return getFlag(F_STAYED_HOME);
+    if (f == "noWorkAgent") return getFlag(F_NO_WORK);
+    if (f == "lastTest") return get32(A.lastTest, N);
+    if (f == "susceptible0") return get32((const uint32_t*)A.susc, N);
+    if (f == "infectedTimestamp") return get32(A.infTs, N);
+    if (f == "infectedLocation") return get32(A.infLoc, N);
+    if (f == "worstStateTimestamp") return get32(A.worstTs, N);
+    if (f == "worstStateEndTimestamp") return get32(A.worstEndTs, N);
+    if (f == "diagnosedTimestamp") return get32(A.diagTs, N);
+    if (f == "quarantinedTimestamp") return get32(A.quarTs, N);
+    if (f == "quarantinedUntilTimestamp") return get32(A.quarUntil, N);
+    if (f == "daysInQuarantine") return get32(A.daysInQ, N);
+    if (f == "hospitalizedTimestamp") return get32(A.hospTs, N);
+    if (f == "hospitalizedUntilTimestamp") return get32(A.hospUntil, N);
+    if (f == "recoveredTimestamp") return get32(A.recTs, N);
+    if (f == "infectedCount") return get8(A.infCount, N, false);
+    if (f == "statsVariant") return get8(A.statVar, N, false);
+    if (f == "worstState") return get8(A.worstState, N, true);
+    if (f == "locQuarantineUntil") return get32(c->L.quarUntil, NL);
+    if (f == "locClosedUntil") return get32(c->L.closedUntil, NL);
+    if (f == "locOpen") return get8(c->L.open, NL, false);
+    if (f == "locInfectiousness") return get32((const uint32_t*)c->L.inf, NL);
+    if (f == "locCount") return get32(c->L.count, NL);
+    return fail(c, PANSIM_E_ARG, "unknown field " + f);
+}
+
+int pansim_device_expf(const float* in, float* out, size_t n) {
+    float *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, n * sizeof(float)) != hipSuccess) return PANSIM_E_DEVICE;
+    if (hipMalloc(&dout, n * sizeof(float)) != hipSuccess) {
+        hipFree(din);
+        return PANSIM_E_DEVICE;
+    }
+    hipMemcpy(din, in, n * sizeof(float), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_expf, dim3(blocks_for(n)), dim3(256), 0, 0, din, dout, n);
+    hipError_t e = hipMemcpy(out, dout, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(din);
+    hipFree(dout);
+    return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+int pansim_device_log(const double* in, double* out, size_t n) {
+    double *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, n * sizeof(double)) != hipSuccess) return PANSIM_E_DEVICE;
+    if (hipMalloc(&dout, n * sizeof(double)) != hipSuccess) {
+        hipFree(din);
+        return PANSIM_E_DEVICE;
+    }
+    hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_log, dim3(blocks_for(n)), dim3(256), 0, 0, din, dout, n);
+    hipError_t e = hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(din);
+    hipFree(dout);
+    return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, size_t n) {
+    uint32_t *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, n * 16) != hipSuccess) return PANSIM_E_DEVICE;
+    if (hipMalloc(&dout, n * 16) != hipSuccess) {
+        hipFree(din);
+        return PANSIM_E_DEVICE;
+    }
+    hipMemcpy(din, ctr, n * 16, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_philox, dim3(blocks_for(n)), dim3(256), 0, 0, din, k0, k1, dout, n);
+    hipError_t e = hipMemcpy(out, dout, n * 16, hipMemcpyDeviceToHost);
+    hipFree(din);
+    hipFree(dout);
+    return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+}  // extern "C"

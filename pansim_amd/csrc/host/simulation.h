@@ -1,0 +1,801 @@
+// Host side of the drop-in: the policy-templated Simulation<...> of the
+// reference (include/simulation.h:53-976) with HIP-backed policies that call
+// the C ABI of include/pansim_hip.h.  Same template parameters, same policy
+// names and hook signatures (addProgramParameters / initializeArgs / init /
+// planLocations / movement / infectionsAtLocations / performTests / midnight /
+// step), same runSimulation and runForDay orderings.  Policies reach the
+// device context through static_cast<SimulationType*>(this)->ctx, as the
+// reference's policies reach agents/locs.
+//
+// Differences on purpose: options come as ps::Options (parsed with the same
+// flag names and defaults as the reference's cxxopts registrations), the
+// population lives in the device context instead of process-wide singletons
+// (so several Simulation objects are independent replicas), and the weekday
+// is (startDay + day) % 7 instead of wall-clock localtime arithmetic.
+#pragma once
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../../include/pansim_hip.h"
+#include "model.h"
+#include "scenario.h"
+
+namespace ps {
+
+// Timehandler (include/tools/timeHandler.h:19-106): a step counter.
+class Timehandler {
+    unsigned counter_ = 0;
+    unsigned timeStep_;
+    unsigned stepsPerDay_;
+    unsigned startDay_;
+    unsigned startDate_;
+
+public:
+    Timehandler(unsigned timeStep, unsigned startDay, unsigned startDate)
+        : timeStep_(timeStep), stepsPerDay_(1440u / timeStep), startDay_(startDay), startDate_(startDate) {}
+    Timehandler& operator++() {
+        ++counter_;
+        return *this;
+    }
+    unsigned getTimestamp() const { return counter_; }
+    unsigned getMinutes() const { return (counter_ % stepsPerDay_) * timeStep_; }
+    unsigned getStepsPerDay() const { return stepsPerDay_; }
+    unsigned getStepsUntilMidnight() const { return stepsPerDay_ - (counter_ % stepsPerDay_); }
+    unsigned getTimeStep() const { return timeStep_; }
+    unsigned getStartDate() const { return startDate_; }
+    bool isMidnight() const { return counter_ % stepsPerDay_ == 0; }
+    unsigned getDay() const { return (startDay_ + counter_ / stepsPerDay_) % 7; }  // 0 = Monday
+};
+
+inline void check(pansim_ctx* ctx, int rc, const char* what) {
+    if (rc != 0) throw InitError(rc, std::string(what) + ": " + pansim_last_error(ctx));
+}
+
+// ---------------------------------------------------------------- movement
+template <typename SimulationType>
+class NoMovement {
+public:
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options&) {}
+    void init(const LocationTypeIds&, unsigned) {}
+    void planLocations(Timehandler, unsigned) {}
+    void movement(Timehandler, unsigned) {}
+};
+
+// RealMovement (include/policies/movementPolicies.h:1339-1752)
+template <typename SimulationType>
+class RealMovement {
+    unsigned quarantineLength = 10;
+
+public:
+    bool enableCurfew = false;
+    unsigned curfewBegin = 0;
+    unsigned curfewEnd = 0;
+    bool curfewTimeConverted = false;
+    unsigned schoolAgeRestriction = 99;
+    bool holidayModeActive = false;
+    bool quarantineImmuneActive = false;
+    bool lockdownNonvaccActive = false;
+    unsigned quarantinePolicy = 3;
+
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options& o) {
+        quarantinePolicy = o.quarantinePolicy;
+        quarantineLength = o.quarantineLength;
+    }
+    void init(const LocationTypeIds&, unsigned) {}
+
+    pansim_step_params stepParams(const Timehandler& t, unsigned timeStep) const {
+        pansim_step_params p{};
+        p.timestamp = t.getTimestamp();
+        p.time_step = timeStep;
+        p.day_of_week = holidayModeActive ? 6u : t.getDay();
+        p.quarantine_policy = quarantinePolicy;
+        p.quarantine_length = quarantineLength;
+        p.enable_curfew = enableCurfew ? 1u : 0u;
+        p.curfew_begin = curfewBegin;
+        p.curfew_end = curfewEnd;
+        p.school_age_restriction = schoolAgeRestriction;
+        p.quarantine_immune_active = quarantineImmuneActive ? 1u : 0u;
+        p.lockdown_nonvacc_active = lockdownNonvaccActive ? 1u : 0u;
+        return p;
+    }
+
+    void planLocations(Timehandler simTime, unsigned timeStep) {
+        auto* self = static_cast<SimulationType*>(this);
+        if (!curfewTimeConverted) {
+            curfewBegin = curfewBegin / timeStep;
+            curfewEnd = curfewEnd / timeStep;
+            curfewTimeConverted = true;
+        }
+        pansim_step_params p = stepParams(simTime, timeStep);
+        check(self->ctx, pansim_plan_locations(self->ctx, &p), "planLocations");
+    }
+
+    void movement(Timehandler simTime, unsigned timeStep) {
+        auto* self = static_cast<SimulationType*>(this);
+        pansim_step_params p = stepParams(simTime, timeStep);
+        check(self->ctx, pansim_movement_step(self->ctx, &p), "movement");
+    }
+};
+
+// ---------------------------------------------------------------- infection
+// BasicInfection (include/policies/infectionPolicies.h:8-589)
+template <typename SimulationType>
+class BasicInfection {
+    double k = 0.000347;
+    double c0_offset = 3.08, trunc = 0.5;
+    std::vector<int> d_offset;
+    int d_peak_offset = 0;
+
+public:
+    static void addProgramParameters(Options&) {}
+    void finalize() {}
+
+    void initializeArgs(const Options& o) {
+        k = o.k;
+        d_offset = split_int(o.d_offset, ',');
+        d_peak_offset = o.d_peak_offset;
+        c0_offset = o.c0;
+        trunc = o.trunc;
+    }
+
+    // infectionPolicies.h:443-478 (host, once per step)
+    double seasonalityMultiplier(const Timehandler& simTime) const {
+        unsigned simDay = simTime.getTimestamp() / simTime.getStepsPerDay();
+        int d = (int)simTime.getStartDate() + (int)simDay;
+        int d_peak = 59 + 22;
+        d += 6;
+        double c0 = 3.08;
+        double trunc_val = 0.5;
+        if (simDay > 250 && simDay < 450) {
+            d += d_offset.empty() ? 0 : d_offset[0];
+            d_peak += d_peak_offset;
+            c0 = c0_offset;
+        } else if (simDay > 600) {
+            trunc_val = trunc;
+            if (d_offset.size() > 1) d += d_offset[1];
+        }
+        int d_mod = d % 366;
+        if (d_mod > 366 / 2) d_mod = 366 - d_mod;
+        double normed = (0.5 * c0 * std::cos(2.0 * M_PI * (double)(d_mod - d_peak) / 366.0) + (1.0 - 0.5 * c0)) /
+                        (0.5 * c0 * std::cos(2.0 * M_PI * (double)(d_peak - d_peak) / 366.0) + (1.0 - 0.5 * c0));
+        double calc = std::min(std::max(normed, trunc_val), 1.0);
+        if (d_mod < d_peak) return 1.0;
+        return calc;
+    }
+
+    // All variants of `mask` in one pass (ascending variant order).
+    void infectionsAtLocationsMask(const Timehandler& simTime, unsigned timeStep, uint32_t mask) {
+        auto* self = static_cast<SimulationType*>(this);
+        float tmpK = (float)k;
+        float season = (float)seasonalityMultiplier(simTime);
+        check(self->ctx, pansim_infection_step(self->ctx, simTime.getTimestamp(), mask, tmpK, season, timeStep),
+              "infectionsAtLocations");
+    }
+    void infectionsAtLocations(const Timehandler& simTime, unsigned timeStep, uint8_t variant) {
+        infectionsAtLocationsMask(simTime, timeStep, 1u << variant);
+    }
+};
+
+// ---------------------------------------------------------------- testing
+template <typename SimulationType>
+class NoTesting {
+public:
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options&) {}
+    void init(const LocationTypeIds&) {}
+    void performTests(Timehandler, unsigned) {}
+    std::tuple<unsigned, unsigned, unsigned> getStats() { return {0u, 0u, 0u}; }
+    void updateTestingProbs(const std::string&) {}
+};
+
+// DetailedTesting (include/policies/testingPolicies.h:292-476)
+template <typename SimulationType>
+class DetailedTesting {
+    unsigned quarantineLength = 10;
+    unsigned stats_[3] = {0, 0, 0};
+    double probs_[6] = {0.005, 0.2, 0.1, 0.1, 0.2, 0.3};
+    unsigned testingDelay = 5;
+    bool usePCR = true;
+
+public:
+    static void addProgramParameters(Options&) {}
+    void updateTestingProbs(const std::string& s) {
+        auto p = split_double(s, ',');
+        for (size_t i = 0; i < p.size() && i < 6; ++i) probs_[i] = p[i];
+    }
+    void initializeArgs(const Options& o) {
+        testingDelay = o.testingRepeatDelay;
+        updateTestingProbs(o.testingProbabilities);
+        quarantineLength = o.quarantineLength;
+        if (o.testingMethod == "PCR") usePCR = true;
+        else if (o.testingMethod == "antigen") usePCR = false;
+        else throw InitError(ERR_BAD_TESTING, "unrecognized testingMethod " + o.testingMethod + " must be either PCR or antigen");
+    }
+    void init(const LocationTypeIds&) {}
+    std::tuple<unsigned, unsigned, unsigned> getStats() { return {stats_[0], stats_[1], stats_[2]}; }
+    void performTests(Timehandler simTime, unsigned timeStep) {
+        auto* self = static_cast<SimulationType*>(this);
+        uint32_t out[3];
+        check(self->ctx,
+              pansim_testing_day(self->ctx, simTime.getTimestamp(), timeStep, probs_, testingDelay, quarantineLength,
+                                 usePCR ? 1u : 0u, out),
+              "performTests");
+        for (int i = 0; i < 3; ++i) stats_[i] = out[i];
+    }
+};
+
+// ---------------------------------------------------------------- closures
+template <typename SimulationType>
+class NoClosure {
+public:
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options&) {}
+    void init(const LocationTypeIds&, const std::vector<ClosureRuleRaw>&, const std::string&) {}
+    void midnight(Timehandler, unsigned, std::vector<unsigned>&) {}
+    void step(Timehandler, unsigned) {}
+};
+
+// RuleClosure (include/policies/closurePolicies.h:47-481)
+template <typename SimulationType>
+class RuleClosure {
+    struct GlobalCondition {
+        std::vector<unsigned> headerPos;
+        std::vector<double> history;
+        int pos = 0;
+        bool active = false;
+        std::function<bool(GlobalCondition*, std::vector<unsigned>&)> condition;
+    };
+    struct Rule {
+        std::string name;
+        std::vector<size_t> conditions;
+        std::function<void(Rule*)> rule;
+        bool previousOpenState = true;
+    };
+    std::vector<GlobalCondition> globalConditions;
+    std::vector<Rule> rules;
+    unsigned enableClosures = 1;
+
+    static std::vector<std::string> splitHeader(const std::string& h) {
+        std::vector<std::string> out;
+        std::string cur;
+        for (char c : h) {
+            if (c == '\t') {
+                if (!cur.empty()) out.push_back(cur);
+                cur.clear();
+            } else cur.push_back(c);
+        }
+        if (!cur.empty()) out.push_back(cur);
+        return out;
+    }
+    bool allActive(const Rule* r) const {
+        bool close = true;
+        for (size_t c : r->conditions) close = close && globalConditions[c].active;
+        return close;
+    }
+
+public:
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options& o) { enableClosures = o.enableClosures; }
+
+    void init(const LocationTypeIds& lt, const std::vector<ClosureRuleRaw>& input, const std::string& statesHeader) {
+        auto* self = static_cast<SimulationType*>(this);
+        std::vector<std::string> header = splitHeader(statesHeader);
+        std::vector<unsigned> hospitalStates;
+        for (unsigned i = 0; i < header.size(); ++i)
+            if (header[i].find("_h") != std::string::npos) hospitalStates.push_back(i);
+        const unsigned numberOfAgents = self->sc.agents.n;
+        static const char* always[] = {"Masks",          "Curfew",           "HolidayMode",      "SchoolAgeRestriction",
+                                       "ExposeToMutation", "ReduceMovement", "QuarantinePolicy", "QuarantineImmune",
+                                       "LockdownNonvacc",  "TestingProb"};
+        globalConditions.reserve(input.size());
+        rules.reserve(input.size());
+        for (const auto& r : input) {
+            bool keep = enableClosures != 0;
+            for (auto* a : always) keep = keep || r.name == a;
+            if (!keep) continue;
+            GlobalCondition c;
+            const int closeAfter = r.closeAfter, openAfter = r.openAfter;
+            const double threshold = r.threshold, threshold2 = r.threshold2;
+            // history window with hysteresis (closurePolicies.h:137-166)
+            auto hysteresis = [threshold, threshold2, openAfter, closeAfter](GlobalCondition* g, double value) {
+                g->history[g->pos] = value;
+                const int hs = (int)g->history.size();
+                g->pos = (g->pos + 1) % hs;
+                auto wrap = [hs](int x) { return ((x % hs) + hs) % hs; };
+                if (!g->active && value > threshold) {
+                    bool below = false;
+                    for (int i = wrap(g->pos - closeAfter); i != g->pos; i = (i + 1) % hs) below |= g->history[i] < threshold;
+                    return below ? g->active : true;
+                }
+                if (g->active && value <= threshold2) {
+                    bool above = false;
+                    for (int i = wrap(g->pos - openAfter); i != g->pos; i = (i + 1) % hs) above |= g->history[i] >= threshold2;
+                    return above ? g->active : false;
+                }
+                return g->active;
+            };
+            if (r.conditionType == "afterDays") {
+                if (closeAfter >= 0 || openAfter <= 0)
+                    throw InitError(ERR_BAD_CLOSURE, "For closure rule 'afterDays', closeAfter must be -1, openAfter must be >0");
+                c.condition = [openAfter, threshold](GlobalCondition* g, std::vector<unsigned>&) {
+                    int day = (int)g->history[0]++;
+                    return day >= threshold && day < threshold + openAfter;
+                };
+                c.history.assign(1, 0.0);
+            } else if (r.conditionType == "hospitalizedFraction") {
+                if (closeAfter <= 0 || openAfter <= 0)
+                    throw InitError(ERR_BAD_CLOSURE, "For closure rule 'hospitalizedFraction', closeAfter and openAfter must be >0");
+                c.headerPos = hospitalStates;
+                c.condition = [hysteresis, numberOfAgents](GlobalCondition* g, std::vector<unsigned>& s) {
+                    double accum = 0.0;
+                    for (unsigned p : g->headerPos) accum += s[p];
+                    return hysteresis(g, accum / (double)numberOfAgents);
+                };
+                c.history.assign(std::max(closeAfter, openAfter) + 2, 0.0);
+            } else if (r.conditionType == "newDeadFraction") {
+                if (closeAfter <= 0 || openAfter <= 0)
+                    throw InitError(ERR_BAD_CLOSURE, "For closure rule 'newDeadFraction', closeAfter and openAfter must be >0");
+                for (unsigned i = 0; i < header.size(); i++)
+                    if (header[i] == "D1") c.headerPos.push_back(i);
+                c.headerPos.push_back(0);
+                c.condition = [hysteresis, numberOfAgents](GlobalCondition* g, std::vector<unsigned>& s) {
+                    double value = (s[g->headerPos[0]] - g->headerPos[1]) / (double)numberOfAgents;
+                    g->headerPos[1] = s[g->headerPos[0]];
+                    return hysteresis(g, value);
+                };
+                c.history.assign(std::max(closeAfter, openAfter) + 2, 0.0);
+            } else {
+                throw InitError(ERR_BAD_CLOSURE, "Unknown closure type " + r.conditionType);
+            }
+            globalConditions.push_back(c);
+            Rule rule;
+            rule.name = r.name;
+            rule.conditions = {globalConditions.size() - 1};
+            if (r.name == "Masks") {
+                double coef = std::atof(r.parameter.c_str());
+                rule.rule = [this, self, coef](Rule* rr) {
+                    bool open = !allActive(rr);
+                    if (rr->previousOpenState != open) {
+                        self->currentMaskValue = open ? 1.0 : coef;
+                        check(self->ctx, pansim_scale_infectiousness(self->ctx, coef, open ? 1u : 0u), "Masks");
+                        rr->previousOpenState = open;
+                    }
+                };
+            } else if (r.name == "Curfew") {
+                unsigned cb = 0, ce = 0;
+                if (r.parameter.size() == 9) {
+                    cb = (unsigned)std::atoi(r.parameter.substr(0, 2).c_str()) * 60 + (unsigned)std::atoi(r.parameter.substr(2, 2).c_str());
+                    ce = (unsigned)std::atoi(r.parameter.substr(5, 2).c_str()) * 60 + (unsigned)std::atoi(r.parameter.substr(7, 2).c_str());
+                } else if (!r.parameter.empty())
+                    throw InitError(ERR_BAD_CLOSURE, "curfew parameter string needs to be exactly 9 characters long");
+                rule.rule = [this, self, cb, ce](Rule* rr) {
+                    bool close = allActive(rr);
+                    if (rr->previousOpenState != !close) {
+                        self->toggleCurfew(close, cb, ce);
+                        rr->previousOpenState = !close;
+                    }
+                };
+            } else if (r.name == "HolidayMode" || r.name == "QuarantineImmune" || r.name == "LockdownNonvacc") {
+                int which = r.name == "HolidayMode" ? 0 : (r.name == "QuarantineImmune" ? 1 : 2);
+                rule.rule = [this, self, which](Rule* rr) {
+                    bool close = allActive(rr);
+                    if (rr->previousOpenState != !close) {
+                        if (which == 0) self->toggleHolidayMode(close);
+                        else if (which == 1) self->toggleQuarantineImmune(close);
+                        else self->toggleLockdownNonvacc(close);
+                        rr->previousOpenState = !close;
+                    }
+                };
+            } else if (r.name == "QuarantinePolicy") {
+                int qp = std::atoi(r.parameter.c_str());
+                rule.rule = [this, self, qp](Rule* rr) {
+                    bool close = allActive(rr);
+                    if (close) {
+                        self->quarantinePolicy((unsigned)qp);
+                        rr->previousOpenState = close;
+                    }
+                };
+            } else if (r.name == "SchoolAgeRestriction") {
+                unsigned limit = (unsigned)std::atoi(r.parameter.c_str());
+                rule.rule = [this, self, limit](Rule* rr) {
+                    bool close = allActive(rr);
+                    if (rr->previousOpenState != !close) {
+                        self->setSchoolAgeRestriction(close ? limit : 99);
+                        rr->previousOpenState = !close;
+                    }
+                };
+            } else if (r.name == "ExposeToMutation") {
+                double fraction = std::atof(r.parameter.c_str());
+                unsigned variantIdx = (unsigned)(int)threshold2;
+                if (variantIdx >= self->sc.variantMultiplier.size())
+                    throw InitError(ERR_BAD_CLOSURE,
+                                    "More ExposeToMutation events defined than infectiousnessMultiplier specified on the command line");
+                rule.rule = [this, self, fraction, variantIdx](Rule* rr) {
+                    if (!allActive(rr)) return;
+                    check(self->ctx,
+                          pansim_expose_mutation(self->ctx, self->getSimTime().getTimestamp(), variantIdx, fraction),
+                          "ExposeToMutation");
+                };
+            } else if (r.name == "ReduceMovement") {
+                double fraction = std::atof(r.parameter.c_str());
+                unsigned homeType = lt.home;
+                rule.rule = [this, self, fraction, homeType](Rule* rr) {
+                    bool open = !allActive(rr);
+                    if (rr->previousOpenState != open) {
+                        self->reduceMovement(fraction, homeType, !open);
+                        rr->previousOpenState = open;
+                    }
+                };
+            } else if (r.name == "TestingProb") {
+                std::string probs = r.parameter;
+                rule.rule = [this, self, probs](Rule* rr) {
+                    if (allActive(rr)) self->updateTestingProbs(probs);
+                };
+            } else {
+                std::vector<uint32_t> toClose;
+                for (int tt : r.locationTypesToClose) toClose.push_back((uint32_t)tt);
+                if (toClose.size() >= 10)
+                    throw InitError(ERR_BAD_CLOSURE, "Error, Closure Rule " + r.name + " has over 10 location types to close");
+                rule.rule = [this, self, toClose](Rule* rr) {
+                    bool open = !allActive(rr);
+                    if (rr->previousOpenState != open) {
+                        check(self->ctx, pansim_set_open_by_type(self->ctx, toClose.data(), (uint32_t)toClose.size(), open ? 1u : 0u),
+                              "closure rule");
+                        rr->previousOpenState = open;
+                    }
+                };
+            }
+            rules.push_back(rule);
+        }
+    }
+
+    void midnight(Timehandler, unsigned, std::vector<unsigned>& stats) {
+        for (auto& c : globalConditions) c.active = c.condition(&c, stats);
+        for (auto& r : rules) r.rule(&r);
+    }
+    void step(Timehandler, unsigned) {}
+};
+
+// ---------------------------------------------------------------- state types
+// Marker types for the PPState / AgentMeta template parameters: the state
+// record (dynamicPPState.h) and agent metadata (agentMeta.h) live in device
+// SoA arrays; these expose their static API.
+struct DynamicPPState {
+    static unsigned getNumberOfStates(const Scenario& sc) { return (unsigned)sc.st.names.size(); }
+    static std::vector<std::string> getStateNames(const Scenario& sc) { return sc.st.names; }
+};
+struct BasicAgentMeta {};
+
+// ---------------------------------------------------------------- simulation
+template <typename PositionType, typename TypeOfLocation, typename PPState, typename AgentMeta,
+          template <typename> class MovementPolicy, template <typename> class InfectionPolicy,
+          template <typename> class TestingPolicy, template <typename> class ClosurePolicy>
+class Simulation
+    : public MovementPolicy<Simulation<PositionType, TypeOfLocation, PPState, AgentMeta, MovementPolicy, InfectionPolicy,
+                                       TestingPolicy, ClosurePolicy>>,
+      public InfectionPolicy<Simulation<PositionType, TypeOfLocation, PPState, AgentMeta, MovementPolicy, InfectionPolicy,
+                                        TestingPolicy, ClosurePolicy>>,
+      public TestingPolicy<Simulation<PositionType, TypeOfLocation, PPState, AgentMeta, MovementPolicy, InfectionPolicy,
+                                      TestingPolicy, ClosurePolicy>>,
+      public ClosurePolicy<Simulation<PositionType, TypeOfLocation, PPState, AgentMeta, MovementPolicy, InfectionPolicy,
+                                      TestingPolicy, ClosurePolicy>> {
+public:
+    using Self = Simulation;
+    using Movement = MovementPolicy<Self>;
+    using Infection = InfectionPolicy<Self>;
+    using Testing = TestingPolicy<Self>;
+    using Closure = ClosurePolicy<Self>;
+
+    Scenario sc;
+    ModelArrays model;
+    pansim_ctx* ctx = nullptr;
+    unsigned timeStep;
+    unsigned lengthOfSimulationWeeks;
+    bool succesfullyInitialized = true;
+    std::string statesHeader;
+    int enableOtherDisease = 1;
+    double currentMaskValue = 1.0;
+    unsigned homeType = 2;
+    unsigned immunizedToday = 0;
+    bool print = true;
+    std::vector<std::vector<unsigned>> dayStats;  // every statistics vector computed at a midnight
+    std::vector<unsigned> variantCounts;
+    std::string initError;
+    int initErrorCode = 0;
+
+    explicit Simulation(const Options& o, bool printOutput = true, uint32_t replica = 0)
+        : timeStep(o.deltat), lengthOfSimulationWeeks(o.weeks), print(printOutput), simTime_(o.deltat ? o.deltat : 10, o.startDay, o.startDate) {
+        try {
+            sc = build_scenario(o);
+            enableOtherDisease = o.otherDisease;
+            Infection::initializeArgs(o);
+            Movement::initializeArgs(o);
+            Testing::initializeArgs(o);
+            Closure::initializeArgs(o);
+            build_model(sc, model);
+            int rc = pansim_create(&ctx, &model.dims, &model.tables, &model.locations, &model.agents, o.seed, replica,
+                                   timeStep, o.quarantinePolicy, o.quarantineLength);
+            if (rc != 0) throw InitError(rc, std::string("pansim_create: ") + pansim_create_error());
+            Movement::init(sc.lt, sc.locs.cemetery);
+            Testing::init(sc.lt);
+            statesHeader = sc.statesHeader;
+            homeType = sc.lt.home;
+            Closure::init(sc.lt, sc.rules, statesHeader);
+            variantCounts.assign(sc.variantMultiplier.size(), 0);
+            schedulesHost_ = sc.sched.events;
+            offsetsHost_ = sc.sched.eventOffset;
+            if (print) std::cout << statesHeader << '\n';
+        } catch (const InitError& e) {
+            succesfullyInitialized = false;
+            initError = e.what();
+            initErrorCode = e.code;
+            if (print) std::cerr << e.what() << '\n';
+        } catch (const std::exception& e) {
+            succesfullyInitialized = false;
+            initError = e.what();
+            initErrorCode = ERR_BAD_DATA;
+            if (print) std::cerr << e.what() << '\n';
+        }
+    }
+    ~Simulation() {
+        if (ctx) pansim_destroy(ctx);
+    }
+    Simulation(const Simulation&) = delete;
+    Simulation& operator=(const Simulation&) = delete;
+
+    Timehandler& getSimTime() { return simTime_; }
+
+    // simulation.h:425-471
+    void updateAgents(Timehandler& t) { check(ctx, pansim_progression_day(ctx, t.getTimestamp(), timeStep), "updateAgents"); }
+
+    // Immunization::update (immunization.h:481-900): the susceptibility pass.
+    // The vaccination and booster rollout (from day immunizationStart /
+    // boosterStart) is not part of this build and raises instead.
+    void immunizationUpdate(Timehandler& t) {
+        auto f = [](const std::string& s) { return split_float(s, ','); };
+        auto pi = f(sc.opt.protectionInfection), piw = f(sc.opt.protectionInfectionWaning);
+        auto psy = f(sc.opt.protectionSymptomatic), psyw = f(sc.opt.protectionSymptomaticWaning);
+        auto ph = f(sc.opt.protectionHospitalization), phw = f(sc.opt.protectionHospitalizationWaning);
+        if (pi.size() < 10 || piw.size() < 10 || psy.size() < 10 || psyw.size() < 10 || ph.size() < 10 || phw.size() < 10)
+            throw InitError(ERR_BAD_ARG, "protection parameters need 10 values");
+        float prot[60];
+        for (int i = 0; i < 10; ++i) {
+            prot[i] = pi[i];
+            prot[10 + i] = piw[i];
+            prot[20 + i] = psy[i];
+            prot[30 + i] = psyw[i];
+            prot[40 + i] = ph[i];
+            prot[50 + i] = phw[i];
+        }
+        int32_t sim[7] = {0};
+        float simMul[7] = {0};
+        auto vs = split_int(sc.opt.variantSimilarity, ',');
+        auto vm = split_float(sc.opt.variantSimilarMultiplier, ',');
+        for (size_t i = 0; i < vs.size() && i < 7; ++i) sim[i] = vs[i];
+        for (size_t i = 0; i < vm.size() && i < 7; ++i) simMul[i] = vm[i];
+        check(ctx, pansim_immunization_day(ctx, t.getTimestamp(), timeStep, prot, sim, simMul), "immunization");
+        unsigned simDay = t.getTimestamp() / t.getStepsPerDay();
+        bool active = simDay >= sc.opt.immunizationStart;
+        for (int b : split_int(sc.opt.boosterStart, ',')) active = active || (int)simDay >= b;
+        if (active)
+            throw InitError(ERR_UNSUPPORTED, "vaccination/booster rollout (day >= immunizationStart/boosterStart) is not implemented");
+        immunizedToday = 0;
+    }
+
+    // simulation.h:169-204 host part + kernel
+    void otherDisease(Timehandler& t) {
+        float probabilityMul = 0.0f;
+        unsigned simDay = t.getTimestamp() / t.getStepsPerDay();
+        if (simDay < sc.totalHospitalizations.size()) {
+            uint32_t inHosp = 0;
+            check(ctx, pansim_count_hospitalized_covid(ctx, t.getTimestamp(), sc.opt.dueWithCOVID, &inHosp), "otherDisease");
+            unsigned target = (unsigned)std::max(0.0, double(sc.totalHospitalizations[simDay]) - double(inHosp) * (9730000.0 / 179500.0));
+            probabilityMul = float(target) / 48000.0;
+        }
+        check(ctx, pansim_other_disease_day(ctx, t.getTimestamp(), timeStep, probabilityMul), "otherDisease");
+    }
+
+    std::vector<unsigned> refreshAndPrintStatistics(Timehandler& t, bool printLine = true) {
+        auto ts = Testing::getStats();
+        uint32_t test[3] = {std::get<0>(ts), std::get<1>(ts), std::get<2>(ts)};
+        std::vector<uint32_t> out(512);
+        size_t len = 0;
+        check(ctx, pansim_stats_day(ctx, t.getTimestamp(), timeStep, test, immunizedToday, out.data(), out.size(), &len), "statistics");
+        out.resize(len);
+        std::vector<unsigned> stats(out.begin(), out.end());
+        if (printLine && print) printStats(stats);
+        return stats;
+    }
+
+    std::vector<unsigned> countVariantCases() {
+        std::vector<uint32_t> out(7, 0);
+        check(ctx, pansim_count_variants(ctx, out.data()), "countVariantCases");
+        return std::vector<unsigned>(out.begin(), out.begin() + sc.variantMultiplier.size());
+    }
+
+    uint32_t activeMask() const {
+        uint32_t m = 0;
+        for (size_t v = 0; v < variantCounts.size(); ++v)
+            if (variantCounts[v] > 0) m |= 1u << v;
+        return m;
+    }
+
+    void midnight() {
+        Timehandler& t = simTime_;
+        if (t.getTimestamp() > 0) Testing::performTests(t, timeStep);
+        if (t.getTimestamp() > 0) updateAgents(t);
+        immunizationUpdate(t);
+        if (enableOtherDisease) otherDisease(t);
+        auto stats = refreshAndPrintStatistics(t);
+        dayStats.push_back(stats);
+        Closure::midnight(t, timeStep, stats);
+        Movement::planLocations(t, timeStep);
+        variantCounts = countVariantCases();
+    }
+
+    // One iteration of runSimulation's loop (simulation.h:904-923).
+    void step() {
+        if (simTime_.isMidnight()) midnight();
+        Movement::movement(simTime_, timeStep);
+        Closure::step(simTime_, timeStep);
+        Infection::infectionsAtLocationsMask(simTime_, timeStep, activeMask());
+        ++simTime_;
+    }
+
+    // simulation.h:899-926
+    void runSimulation() {
+        if (!succesfullyInitialized) return;
+        const unsigned total = lengthOfSimulationWeeks * 7u * simTime_.getStepsPerDay();
+        while (simTime_.getTimestamp() < total) step();
+        Infection::finalize();
+    }
+
+    // simulation.h:928-956
+    std::vector<unsigned> runForDay(const std::vector<std::string>& flags) {
+        processFlags(flags);
+        auto stats = refreshAndPrintStatistics(simTime_, false);
+        Closure::midnight(simTime_, timeStep, stats);
+        Movement::planLocations(simTime_, timeStep);
+        variantCounts = countVariantCases();
+        const unsigned spd = simTime_.getStepsPerDay();
+        for (unsigned i = 0; i < spd; ++i) {
+            Movement::movement(simTime_, timeStep);
+            Closure::step(simTime_, timeStep);
+            Infection::infectionsAtLocationsMask(simTime_, timeStep, activeMask());
+            ++simTime_;
+        }
+        if (simTime_.getTimestamp() > 0) Testing::performTests(simTime_, timeStep);
+        if (simTime_.getTimestamp() > 0) updateAgents(simTime_);
+        immunizationUpdate(simTime_);
+        if (enableOtherDisease) otherDisease(simTime_);
+        stats = refreshAndPrintStatistics(simTime_);
+        dayStats.push_back(stats);
+        check(ctx, pansim_rebuild_locations(ctx), "runForDay");
+        return stats;
+    }
+
+    // simulation.h:779-897
+    void processFlags(const std::vector<std::string>& flags) {
+        for (const auto& flag : flags) {
+            std::string prefix = flag.substr(0, 2);
+            if (prefix == "TP") {
+                if (flag == "TPdef") updateTestingProbs("0.00005,0.01,0.0005,0.0005,0.005,0.05");
+                else if (flag == "TP015") updateTestingProbs("0.00005,0.2,0.04,0.04,0.005,0.05");
+                else if (flag == "TP035") updateTestingProbs("0.00005,1.0,0.2,0.2,0.005,0.05");
+            } else if (prefix == "PL") {
+                const uint32_t fix[4] = {5, 6, 22, 44};
+                if (flag == "PLNONE") check(ctx, pansim_set_open_by_type(ctx, fix, 4, 1), "PLNONE");
+                else if (flag == "PL0") check(ctx, pansim_set_open_by_type(ctx, fix, 4, 0), "PL0");
+            } else if (prefix == "CF") {
+                if (flag == "CFNONE") toggleCurfew(false, 0, 0);
+                else if (flag == "CF2000-0500") toggleCurfew(true, 20 * 60, 5 * 60);
+            } else if (prefix == "SO") {
+                if (flag == "SONONE") setSchoolAgeRestriction(99);
+                else if (flag == "SO12") setSchoolAgeRestriction(12);
+                else if (flag == "SO3") setSchoolAgeRestriction(3);
+            } else if (prefix == "QU") {
+                if (flag == "QU0") quarantinePolicy(0);
+                else if (flag == "QU1") quarantinePolicy(1);
+                else if (flag == "QU2") quarantinePolicy(2);
+                else if (flag == "QU3") quarantinePolicy(3);
+            } else if (prefix == "MA") {
+                if (flag == "MA1.0" && currentMaskValue != 1.0) {
+                    check(ctx, pansim_scale_infectiousness(ctx, currentMaskValue, 1), "MA1.0");
+                    currentMaskValue = 1.0;
+                } else if (flag == "MA0.8" && currentMaskValue != 0.8) {
+                    currentMaskValue = 0.8;
+                    check(ctx, pansim_scale_infectiousness(ctx, 0.8, 0), "MA0.8");
+                }
+            }
+        }
+    }
+
+    void toggleCurfew(bool enable, unsigned begin, unsigned end) {
+        Movement::enableCurfew = enable;
+        Movement::curfewBegin = begin;
+        Movement::curfewEnd = end;
+        Movement::curfewTimeConverted = false;
+    }
+    void setSchoolAgeRestriction(unsigned limit) { Movement::schoolAgeRestriction = limit; }
+    void toggleHolidayMode(bool enable) { Movement::holidayModeActive = enable; }
+    void toggleQuarantineImmune(bool enable) { Movement::quarantineImmuneActive = enable; }
+    void toggleLockdownNonvacc(bool enable) { Movement::lockdownNonvaccActive = enable; }
+    void quarantinePolicy(unsigned qp) { Movement::quarantinePolicy = qp; }
+    void updateTestingProbs(const std::string& probs) { Testing::updateTestingProbs(probs); }
+
+    // AgentTypeList::setStayHome / unsetStayHome (src/agentData/agentType.cpp:42-109)
+    void reduceMovement(double probability, unsigned homeT, bool set) {
+        auto& off = offsetsHost_;
+        auto& ev = schedulesHost_;
+        auto same = [](const Event& a, const Event& b) { return a.start_h == b.start_h && a.start_m == b.start_m; };
+        for (unsigned s = 0; s + 1 < off.size(); s++) {
+            unsigned ws = off[s], we = ws;
+            while (ws < off[s + 1]) {
+                while (we < ev.size() && same(ev[we], ev[ws])) we++;
+                if (!set) {
+                    for (unsigned i = ws; i < we; i++)
+                        if (ev[i].locType != homeT) ev[i].chance = (float)(ev[i].chance / (1.0 - probability));
+                    for (unsigned i = ws; i < we; i++)
+                        if (ev[i].locType == homeT) {
+                            ev[i].chance = (float)((ev[i].chance - probability) / (1.0 - probability));
+                            if (std::abs(ev[i].chance) < 1e-10) {
+                                ev.erase(ev.begin() + i);
+                                for (size_t j = s + 1; j < off.size(); j++) off[j]--;
+                            }
+                            break;
+                        }
+                } else {
+                    bool home = false;
+                    for (unsigned i = ws; i < we; i++)
+                        if (ev[i].locType == homeT) {
+                            home = true;
+                            ev[i].chance = (float)(ev[i].chance + (1.0 - ev[i].chance) * probability);
+                            break;
+                        }
+                    if (home && we - ws == 1) {
+                        ws = we;
+                        continue;
+                    }
+                    if (!home) {
+                        Event h = ev[ws];
+                        h.locType = homeT;
+                        h.chance = (float)probability;
+                        ev.insert(ev.begin() + ws, h);
+                        we++;
+                        for (size_t j = s + 1; j < off.size(); j++) off[j]++;
+                    }
+                    for (unsigned i = ws; i < we; i++)
+                        if (ev[i].locType != homeT) ev[i].chance = (float)(ev[i].chance * (1.0 - probability));
+                }
+                ws = we;
+            }
+        }
+        auto cev = to_c_events(ev);
+        check(ctx, pansim_upload_schedules(ctx, off.data(), (uint32_t)off.size(), cev.data(), (uint32_t)cev.size()),
+              "ReduceMovement");
+    }
+
+    void printStats(const std::vector<unsigned>& s) const {
+        for (size_t i = 0; i < s.size(); ++i) std::cout << s[i] << (i + 1 < s.size() ? "\t" : "\n");
+    }
+
+private:
+    Timehandler simTime_;
+    std::vector<Event> schedulesHost_;
+    std::vector<uint32_t> offsetsHost_;
+};
+
+// The reference's concrete binding (cmake/config/in/configTypes.h.in:10-16).
+using ConfiguredSimulation = Simulation<std::array<double, 2>, unsigned, DynamicPPState, BasicAgentMeta, RealMovement,
+                                        BasicInfection, DetailedTesting, RuleClosure>;
+
+}  // namespace ps

@@ -1,0 +1,70 @@
+"""GPU parity: the HIP path against the CPU oracle, bit-exact on every agent
+and location field, the per-location infectiousness sums and the per-day
+statistics vectors.  Small populations (seconds on the oracle), high
+infection coefficients so that every state of the progression, testing,
+quarantine and hospital paths is reached within the run."""
+import json
+import os
+
+import pytest
+
+import pansim_amd
+from tests.parity_util import run_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(n=3000, nl=600, weeks=2, seed=0, k="0.02", extra=()):
+    return pansim_amd.config1_args(n, nl, weeks, seed, extra=["-k", k, *extra])
+
+
+def test_parity_config1_small():
+    # midnight, mid-day and end-of-day checkpoints over two weeks
+    run_parity(_cfg(), [1, 37, 144, 145, 300, 1000, 2016])
+
+
+@pytest.mark.parametrize("qp", [0, 1, 2, 4])
+def test_parity_quarantine_policies(qp):
+    run_parity(_cfg(n=2500, nl=500, weeks=1, seed=qp + 1, extra=["--quarantinePolicy", str(qp)]), [100, 500, 1008])
+
+
+def test_parity_default_closure_rules_and_antigen():
+    args = pansim_amd.config1_args(2500, 500, 1, 7, extra=[
+        "-k", "0.03", "--enableClosures", "1", "--closures", "inputConfigFiles/closureRules.json",
+        "--testingMethod", "antigen", "--testingProbabilities", "0.01,0.5,0.2,0.2,0.05,0.3"])
+    run_parity(args, [144, 700, 1008])
+
+
+def test_parity_run_for_day_flags():
+    args = _cfg(n=2500, nl=500, weeks=1, seed=11, k="0.03")
+    days = [("day", []), ("day", ["CF2000-0500", "TP015"]), ("day", ["PL0", "SO12", "MA0.8"]),
+            ("day", ["QU2", "TP035"]), ("day", ["CFNONE", "PLNONE", "MA1.0", "QU3", "SONONE"]), ("day", [])]
+    run_parity(args, days)
+
+
+def test_parity_synthetic_population():
+    args = pansim_amd.synthetic_args(6000, 1, 3, 5, extra=["-k", "0.01", "--synthInfected", "0.01"])
+    run_parity(args, [60, 144, 600, 1008])
+
+
+def test_parity_mutation_and_rules(tmp_path):
+    rules = {"rules": [
+        {"name": "ExposeToMutation", "conditionType": "afterDays", "threshold": 1, "threshold2": 2,
+         "parameter": "0.02", "openAfter": 2, "closeAfter": -1, "locationTypesToClose": []},
+        {"name": "Masks", "conditionType": "afterDays", "threshold": 2, "threshold2": 0,
+         "parameter": "0.6", "openAfter": 3, "closeAfter": -1, "locationTypesToClose": []},
+        {"name": "Curfew", "conditionType": "afterDays", "threshold": 3, "threshold2": 0,
+         "parameter": "2100-0600", "openAfter": 2, "closeAfter": -1, "locationTypesToClose": []},
+        {"name": "schools", "conditionType": "afterDays", "threshold": 1, "threshold2": 0,
+         "parameter": "0", "openAfter": 3, "closeAfter": -1, "locationTypesToClose": [3, 33, 8]},
+        {"name": "ReduceMovement", "conditionType": "afterDays", "threshold": 2, "threshold2": 0,
+         "parameter": "0.3", "openAfter": 2, "closeAfter": -1, "locationTypesToClose": []},
+        {"name": "HolidayMode", "conditionType": "afterDays", "threshold": 5, "threshold2": 0,
+         "parameter": "0", "openAfter": 1, "closeAfter": -1, "locationTypesToClose": []},
+        {"name": "hosp", "conditionType": "hospitalizedFraction", "threshold": 0.0001, "threshold2": 0.00005,
+         "parameter": "0", "openAfter": 2, "closeAfter": 1, "locationTypesToClose": [5, 6]},
+    ]}
+    p = tmp_path / "rules.json"
+    p.write_text(json.dumps(rules))
+    args = pansim_amd.config1_args(2500, 500, 1, 21, extra=["-k", "0.03", "--closures", str(p), "--enableClosures", "1"])
+    run_parity(args, [144, 300, 600, 1008])
