@@ -1898,6 +1898,34 @@ int oracle_location_sums(void* h, unsigned v, long long* out) {
     return 0;
 }
 
+// Structure accessors for the invariant tests.
+unsigned oracle_csr_size(void* h) { return (unsigned)((oracle::Sim*)h)->possibleLocations.size(); }
+void oracle_csr(void* h, unsigned* offsets, unsigned* locs, unsigned char* types) {
+    auto* s = (oracle::Sim*)h;
+    for (size_t i = 0; i < s->locOffset.size(); ++i) offsets[i] = s->locOffset[i];
+    for (size_t i = 0; i < s->possibleLocations.size(); ++i) {
+        locs[i] = s->possibleLocations[i];
+        types[i] = s->possibleTypes[i];
+    }
+}
+unsigned oracle_cemetery(void* h) { return ((oracle::Sim*)h)->cemetery; }
+void oracle_location_types(void* h, unsigned* out) {
+    auto* s = (oracle::Sim*)h;
+    for (unsigned l = 0; l < s->L; ++l) out[l] = s->locType[l];
+}
+unsigned oracle_num_states(void* h) { return ((oracle::Sim*)h)->nStates; }
+void oracle_state_table(void* h, unsigned char* wb, float* infectious) {
+    auto* s = (oracle::Sim*)h;
+    for (unsigned k = 0; k < s->nStates; ++k) {
+        wb[k] = s->wb[k];
+        infectious[k] = s->infTable[k];
+    }
+}
+void oracle_agent_infectious(void* h, float* out) {
+    auto* s = (oracle::Sim*)h;
+    for (unsigned i = 0; i < s->N; ++i) out[i] = s->pp[i].infectious;
+}
+
 // Known-answer helpers for the tests.
 void oracle_philox(const unsigned* ctr, unsigned k0, unsigned k1, unsigned* out) { oracle::philox10(ctr, k0, k1, out); }
 float oracle_expf(float x) { return oracle::exp_f32(x); }

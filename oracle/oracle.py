@@ -39,6 +39,16 @@ def lib():
     L.oracle_day_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p, ctypes.c_int]
     L.oracle_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, u32p]
     L.oracle_location_sums.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_longlong)]
+    L.oracle_csr_size.argtypes = [ctypes.c_void_p]
+    L.oracle_csr_size.restype = ctypes.c_uint32
+    L.oracle_csr.argtypes = [ctypes.c_void_p, u32p, u32p, ctypes.POINTER(ctypes.c_uint8)]
+    L.oracle_cemetery.argtypes = [ctypes.c_void_p]
+    L.oracle_cemetery.restype = ctypes.c_uint32
+    L.oracle_location_types.argtypes = [ctypes.c_void_p, u32p]
+    L.oracle_num_states.argtypes = [ctypes.c_void_p]
+    L.oracle_num_states.restype = ctypes.c_uint32
+    L.oracle_state_table.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)]
+    L.oracle_agent_infectious.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
     L.oracle_philox.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
     L.oracle_expf.argtypes = [ctypes.c_float]
     L.oracle_expf.restype = ctypes.c_float
@@ -123,6 +133,42 @@ class OracleSim:
 
     def seasonality(self, ts: int) -> float:
         return lib().oracle_seasonality(self._h, ts)
+
+    def csr(self):
+        """(offsets[N+1], possible locations, their location types) of every agent."""
+        m = lib().oracle_csr_size(self._h)
+        off = np.zeros(self.n_agents + 1, dtype=np.uint32)
+        locs = np.zeros(max(1, m), dtype=np.uint32)
+        types = np.zeros(max(1, m), dtype=np.uint8)
+        lib().oracle_csr(self._h, _u32(off), _u32(locs), types.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return off, locs[:m], types[:m]
+
+    @property
+    def cemetery(self) -> int:
+        return lib().oracle_cemetery(self._h)
+
+    def location_types(self) -> np.ndarray:
+        out = np.zeros(self.n_locations, dtype=np.uint32)
+        lib().oracle_location_types(self._h, _u32(out))
+        return out
+
+    def state_table(self):
+        """(wellbeing code, infectiousness) of every disease state."""
+        n = lib().oracle_num_states(self._h)
+        wb = np.zeros(n, dtype=np.uint8)
+        inf = np.zeros(n, dtype=np.float32)
+        lib().oracle_state_table(self._h, wb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                 inf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return wb, inf
+
+    def agent_infectious(self) -> np.ndarray:
+        out = np.zeros(self.n_agents, dtype=np.float32)
+        lib().oracle_agent_infectious(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out
+
+
+def _u32(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
 
 
 def philox(ctr, key):

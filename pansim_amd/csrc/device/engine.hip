@@ -34,7 +34,8 @@ struct pansim_ctx {
     // per-call HIP event pairs on the context stream while timing is on
     std::vector<hipEvent_t> moveEv, infectEv;
     size_t moveN = 0, infectN = 0;
-    unsigned long long* moveCounters = nullptr;  // evaluated movers, relocated agents
+    unsigned long long* moveCounters = nullptr;  // MOVE_CTR_SLOTS pairs (evaluated movers, relocated agents)
+    uint32_t pendPar = 0;                         // parity of the pending-write counter of the next step
     // schedule storage (re-uploadable)
     uint32_t* evOffDev = nullptr;
     pansim_event* evDev = nullptr;
@@ -247,21 +248,18 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(upload(c, (uint32_t**)&A.possLoc, a->possible_locations, d->n_csr));
     TRY(upload(c, (uint8_t**)&A.possType, a->possible_types, d->n_csr));
     // ---- scratch
-    TRY(dalloc(c, &c->S.queue, N));
-    TRY(dalloc(c, &c->S.queueCount, 1));
-    c->S.pendCap = 3u * N + 16u;
+    c->S.pendCap = d->n_locations + 16u;
     TRY(dalloc(c, &c->S.pendList, c->S.pendCap));
-    TRY(dalloc(c, &c->S.pendCount, 1));
-    HIP_OR_FAIL(c, hipMemsetAsync(c->S.queueCount, 0, sizeof(uint32_t), c->stream));
-    HIP_OR_FAIL(c, hipMemsetAsync(c->S.pendCount, 0, sizeof(uint32_t), c->stream));
+    TRY(dalloc(c, &c->S.pendCount, 2));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &c->counters, S_COUNT + 16));
     if (hipHostMalloc((void**)&c->hostCounters, (S_COUNT + 16) * sizeof(unsigned long long), hipHostMallocDefault) !=
         hipSuccess) {
         c->err = "hipHostMalloc failed";
         return bail(PANSIM_E_DEVICE);
     }
-    TRY(dalloc(c, &c->moveCounters, 2));
-    HIP_OR_FAIL(c, hipMemsetAsync(c->moveCounters, 0, 2 * sizeof(unsigned long long), c->stream));
+    TRY(dalloc(c, &c->moveCounters, 2 * MOVE_CTR_SLOTS));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->moveCounters, 0, 2 * MOVE_CTR_SLOTS * sizeof(unsigned long long), c->stream));
     // ---- initial agent state
     uint32_t qlenSteps = quarantine_length * 24u * 60u / time_step;
     hipLaunchKernelGGL(k_init_agents, dim3(blocks_for(N)), dim3(256), 0, c->stream, c->A, c->T, seed, replica,
@@ -346,11 +344,16 @@ int pansim_last_kernel_ms(pansim_ctx* c, float* move_ms, float* infect_ms) {
 
 int pansim_movement_counters(pansim_ctx* c, uint64_t* evaluated, uint64_t* relocated, uint32_t reset) {
     if (!c) return PANSIM_E_ARG;
-    unsigned long long h[2];
+    unsigned long long h[2 * MOVE_CTR_SLOTS];
     HIP_OR_FAIL(c, hipMemcpyAsync(h, c->moveCounters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    if (evaluated) *evaluated = h[0];
-    if (relocated) *relocated = h[1];
+    uint64_t ev = 0, re = 0;
+    for (uint32_t s = 0; s < MOVE_CTR_SLOTS; ++s) {
+        ev += h[2 * s];
+        re += h[2 * s + 1];
+    }
+    if (evaluated) *evaluated = ev;
+    if (relocated) *relocated = re;
     if (reset) HIP_OR_FAIL(c, hipMemsetAsync(c->moveCounters, 0, sizeof(h), c->stream));
     return 0;
 }
@@ -369,12 +372,11 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     m.curfewStep = (p->enable_curfew && p->curfew_begin == minutes / p->time_step) ? 1u : 0u;
     m.nEvents = c->nEvents;
     m.counters = c->moveCounters;
+    m.pendPar = c->pendPar;
+    c->pendPar ^= 1u;
     if (c->timing && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
-    hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(64), 0, c->stream, c->S.queueCount, c->S.pendCount);
-    hipLaunchKernelGGL(k_move_scan, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A.wake, c->A.n, p->timestamp,
-                       m.curfewStep, c->S.queue, c->S.queueCount);
-    hipLaunchKernelGGL(k_move, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, m);
-    hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, c->moveCounters);
+    hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
+    hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, m.pendPar);
     if (c->timing) {
         if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
         c->moveN++;
@@ -396,9 +398,9 @@ int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, floa
     a.k = k;
     a.season = season;
     a.dt = (float)dt;
-    unsigned grid = blocks_for(c->A.n);
+    unsigned grid = blocks_for(c->A.n, INFECT_TILE);
     if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
-    hipLaunchKernelGGL(k_infect, dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_infect, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
     if (c->timing) {
         if (record_event(c, c->infectEv, 2 * c->infectN + 1)) return PANSIM_E_DEVICE;
         c->infectN++;
@@ -702,6 +704,8 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
 }
 
 int pansim_device_expf(const float* in, float* out, size_t n) {
+    if (n == 0) return 0;
+    if (!in || !out) return PANSIM_E_ARG;
     float *din = nullptr, *dout = nullptr;
     if (hipMalloc(&din, n * sizeof(float)) != hipSuccess) return PANSIM_E_DEVICE;
     if (hipMalloc(&dout, n * sizeof(float)) != hipSuccess) {
@@ -717,6 +721,8 @@ int pansim_device_expf(const float* in, float* out, size_t n) {
 }
 
 int pansim_device_log(const double* in, double* out, size_t n) {
+    if (n == 0) return 0;
+    if (!in || !out) return PANSIM_E_ARG;
     double *din = nullptr, *dout = nullptr;
     if (hipMalloc(&din, n * sizeof(double)) != hipSuccess) return PANSIM_E_DEVICE;
     if (hipMalloc(&dout, n * sizeof(double)) != hipSuccess) {
@@ -732,6 +738,8 @@ int pansim_device_log(const double* in, double* out, size_t n) {
 }
 
 int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, size_t n) {
+    if (n == 0) return 0;
+    if (!ctr || !out) return PANSIM_E_ARG;
     uint32_t *din = nullptr, *dout = nullptr;
     if (hipMalloc(&din, n * 16) != hipSuccess) return PANSIM_E_DEVICE;
     if (hipMalloc(&dout, n * 16) != hipSuccess) {

@@ -120,10 +120,8 @@ struct Tables {
 
 // Per-step scratch shared by the movement and infection kernels.
 struct StepScratch {
-    uint32_t* queue;        // movers of this step
-    uint32_t* queueCount;
     uint32_t* pendList;     // locations with a pending quarantine write
-    uint32_t* pendCount;
+    uint32_t* pendCount;    // [2]: entries of pendList, by step parity
     uint32_t pendCap;
 };
 

@@ -106,14 +106,20 @@ struct MoveArgs {
     uint32_t replica;
     uint32_t curfewStep;  // all agents take the full path this step
     uint32_t nEvents;
-    unsigned long long* counters;  // [1] = agents that changed location
+    uint32_t pendPar;  // parity of the pending-write counter used this step
+    unsigned long long* counters;  // MOVE_CTR_SLOTS pairs (evaluated, relocated)
 };
+
+constexpr uint32_t MOVE_BLOCK = 256;
+constexpr uint32_t MOVE_ITEMS = 16;
+constexpr uint32_t MOVE_TILE = MOVE_BLOCK * MOVE_ITEMS;  // agents per block
+constexpr uint32_t MOVE_CTR_SLOTS = 64;
 
 __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, uint32_t loc, uint32_t until) {
     unsigned long long key = ((unsigned long long)agent << 32) | (unsigned long long)until;
     unsigned long long old = atomicMax(&m.L.pendKey[loc], key);
     if (old == 0ull) {
-        uint32_t slot = atomicAdd(m.S.pendCount, 1u);
+        uint32_t slot = atomicAdd(&m.S.pendCount[m.pendPar], 1u);
         if (slot < m.S.pendCap) m.S.pendList[slot] = loc;
     }
 }
@@ -152,8 +158,9 @@ __device__ void quarantine_agent(const MoveArgs& m, uint32_t i, uint32_t until, 
 
 // RealMovementOps::doMovement (movementPolicies.h:258-1041) for agent i.
 // Writes location, wake step, flags and stats of agent i; keeps the
-// per-location occupancy and infectiousness sums up to date.
-__device__ void do_movement(const MoveArgs& m, uint32_t i) {
+// per-location occupancy and infectiousness sums up to date.  Returns
+// whether the agent changed location.
+__device__ bool do_movement(const MoveArgs& m, uint32_t i) {
     const Agents& A = m.A;
     const Locs& L = m.L;
     const Tables& T = m.T;
@@ -401,60 +408,78 @@ __device__ void do_movement(const MoveArgs& m, uint32_t i) {
 
     if (fl != fl0) A.flags[i] = fl;
     if (wake != A.wake[i]) A.wake[i] = wake;
-    unsigned long long moved = __ballot(loc != oldLoc);
-    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)__ballot(1)) - 1) && moved)
-        atomicAdd(&m.counters[1], (unsigned long long)__popcll(moved));
-    if (loc != oldLoc) {
-        A.location[i] = loc;
-        atomicSub(&L.count[oldLoc], 1u);
-        atomicAdd(&L.count[loc], 1u);
-        float x = T.stInf[st] * T.varMul[A.variant[i]];
-        if (x != 0.0f) {
-            long long q = fixed32(x);
-            long long* s = L.sum + (size_t)A.variant[i] * L.n;
-            atomicAdd((unsigned long long*)&s[oldLoc], (unsigned long long)(-q));
-            atomicAdd((unsigned long long*)&s[loc], (unsigned long long)q);
+    if (loc == oldLoc) return false;
+    A.location[i] = loc;
+    atomicSub(&L.count[oldLoc], 1u);
+    atomicAdd(&L.count[loc], 1u);
+    float x = T.stInf[st] * T.varMul[A.variant[i]];
+    if (x != 0.0f) {
+        long long q = fixed32(x);
+        long long* s = L.sum + (size_t)A.variant[i] * L.n;
+        atomicAdd((unsigned long long*)&s[oldLoc], (unsigned long long)(-q));
+        atomicAdd((unsigned long long*)&s[loc], (unsigned long long)q);
+    }
+    return true;
+}
+
+// One movement step over a tile of MOVE_TILE consecutive agents per block:
+// the agents whose countdown reaches zero at this step (everyone on the
+// curfew step) are compacted into LDS, then processed by the same block.
+// The order in which movers are processed does not change any result: the
+// per-agent writes are private, the per-location sums are integer atomics and
+// the location quarantine writes are deferred to k_resolve_pending.
+// Evaluated / relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
+__global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) {
+    __shared__ uint32_t list[MOVE_TILE];
+    __shared__ uint32_t nList, nMoved;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t base = blockIdx.x * MOVE_TILE;
+    const uint32_t n = m.A.n;
+    const uint32_t ts = m.p.timestamp;
+    if (tid == 0) {
+        nList = 0;
+        nMoved = 0;
+    }
+    uint32_t w[MOVE_ITEMS];
+#pragma unroll
+    for (uint32_t k = 0; k < MOVE_ITEMS; ++k) {
+        uint32_t i = base + k * MOVE_BLOCK + tid;
+        w[k] = i < n ? __builtin_nontemporal_load(&m.A.wake[i]) : ts + 1u;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (uint32_t k = 0; k < MOVE_ITEMS; ++k) {
+        uint32_t i = base + k * MOVE_BLOCK + tid;
+        bool mover = i < n && (m.curfewStep || w[k] == ts);
+        unsigned long long mask = __ballot(mover);
+        if (mask != 0ull) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
+            pos = __shfl(pos, 0, 64);
+            if (mover) list[pos + (uint32_t)__popcll(mask & below)] = i;
         }
     }
-}
-
-// Mover scan: agents whose countdown reaches zero at this step (or everyone
-// on the curfew step) are appended to the queue with one atomic per wave.
-__global__ void k_reset_step(uint32_t* __restrict__ queueCount, uint32_t* __restrict__ pendCount) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *queueCount = 0u;
-        *pendCount = 0u;
+    __syncthreads();
+    const uint32_t cnt = nList;
+    uint32_t moved = 0;
+    for (uint32_t j = tid; j < cnt; j += MOVE_BLOCK) moved += do_movement(m, list[j]) ? 1u : 0u;
+    if (moved) atomicAdd(&nMoved, moved);
+    __syncthreads();
+    if (tid == 0 && cnt) {
+        unsigned long long* c = m.counters + 2u * (blockIdx.x % MOVE_CTR_SLOTS);
+        atomicAdd(&c[0], (unsigned long long)cnt);
+        if (nMoved) atomicAdd(&c[1], (unsigned long long)nMoved);
     }
-}
-
-__global__ void __launch_bounds__(256) k_move_scan(const uint32_t* __restrict__ wake, uint32_t n, uint32_t ts,
-                                                   uint32_t all, uint32_t* __restrict__ queue,
-                                                   uint32_t* __restrict__ queueCount) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool mover = false;
-    if (i < n) mover = all || __builtin_nontemporal_load(&wake[i]) == ts;
-    unsigned long long mask = __ballot(mover);
-    if (mask == 0ull) return;
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (lane == (uint32_t)(__ffsll((long long)mask) - 1)) base = atomicAdd(queueCount, (uint32_t)__popcll(mask));
-    base = __shfl(base, __ffsll((long long)mask) - 1, 64);
-    if (mover) {
-        uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        queue[base + rank] = i;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_move(MoveArgs m) {
-    uint32_t q = *m.S.queueCount;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < q; j += gridDim.x * blockDim.x) do_movement(m, m.S.queue[j]);
 }
 
 // End of the step's movement: the deferred quarantine writes; the highest
-// writing agent index of each location wins.
-__global__ void __launch_bounds__(256) k_resolve_pending(Locs L, StepScratch S, unsigned long long* counters) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&counters[0], (unsigned long long)*S.queueCount);
-    uint32_t pc = *S.pendCount;
+// writing agent index of each location wins.  The step's pending list count
+// is pendCount[par]; the other parity's count is cleared for the next step.
+__global__ void __launch_bounds__(256) k_resolve_pending(Locs L, StepScratch S, uint32_t par) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.pendCount[par ^ 1u] = 0u;
+    uint32_t pc = S.pendCount[par];
     if (pc > S.pendCap) pc = S.pendCap;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < pc; j += gridDim.x * blockDim.x) {
         uint32_t loc = S.pendList[j];
@@ -477,45 +502,66 @@ struct InfectArgs {
     float dt;
 };
 
-// Infection of all active variants + end-of-step quarantine writes.
-// LocationsList::infectAgents (locationList.h:251-305) with the ratio of
-// infectionPolicies.h:523-546 evaluated from the location's sums.
-__global__ void __launch_bounds__(256) k_infect(InfectArgs a) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride = gridDim.x * blockDim.x;
+// Infection of all active variants.  LocationsList::infectAgents
+// (locationList.h:251-305) with the ratio of infectionPolicies.h:523-546
+// evaluated from the location's exact sums.  Each thread takes INFECT_ITEMS
+// agents (stride INFECT_BLOCK) and issues their independent loads together,
+// so that a wave keeps several memory round trips in flight.
+constexpr uint32_t INFECT_BLOCK = 256;
+constexpr uint32_t INFECT_ITEMS = 4;
+constexpr uint32_t INFECT_TILE = INFECT_BLOCK * INFECT_ITEMS;
+
+__device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
+                                             float s) {
     const uint32_t n = a.A.n;
+    float num = (float)((double)e * (1.0 / 4294967296.0));
+    float density = num / (float)a.L.count[loc];
+    float ratio = 1.0f - ps_expf(-a.k * density * a.L.inf[loc] * a.season * a.dt);
+    if (ratio == 0.0f) return;
+    Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
+    if (rs.unit() < (double)(ratio * s)) {
+        // gotInfected (dynamicPPState.cpp:235-244) + stats (locationList.h:286-291)
+        uint8_t fi = (uint8_t)a.T.firstInfected;
+        a.A.state[i] = fi;
+        a.A.variant[i] = (uint8_t)v;
+        a.A.days[i] = -2;
+        float sf = a.T.stSusc[fi] ? 1.0f : 0.0f;
+        for (uint32_t u = 0; u < a.T.nVar; ++u) a.A.susc[(size_t)u * n + i] = sf;
+        a.A.infTs[i] = a.timestamp;
+        a.A.infCount[i] = (uint8_t)(a.A.infCount[i] + 1);
+        a.A.infLoc[i] = loc;
+        a.A.worstState[i] = (int8_t)fi;
+        a.A.worstTs[i] = a.timestamp;
+        a.A.statVar[i] = (uint8_t)(a.A.statVar[i] | (1u << v));
+    }
+}
+
+__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
     if (a.mask == 0u) return;
-    for (uint32_t i = gid; i < n; i += stride) {
-        uint32_t loc = NOLOC;
-        for (uint32_t v = 0; v < a.T.nVar; ++v) {
-            if (!((a.mask >> v) & 1u)) continue;
-            float s = a.A.susc[(size_t)v * n + i];
-            if (!(s > 0.0f)) continue;
-            if (loc == NOLOC) loc = a.A.location[i];
-            long long e = a.L.sum[(size_t)v * a.L.n + loc];
-            if (e == 0) continue;
-            float num = (float)((double)e * (1.0 / 4294967296.0));
-            float density = num / (float)a.L.count[loc];
-            float ratio = 1.0f - ps_expf(-a.k * density * a.L.inf[loc] * a.season * a.dt);
-            if (ratio == 0.0f) continue;
-            Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
-            if (rs.unit() < (double)(ratio * s)) {
-                // gotInfected (dynamicPPState.cpp:235-244) + stats (locationList.h:286-291)
-                uint8_t fi = (uint8_t)a.T.firstInfected;
-                a.A.state[i] = fi;
-                a.A.variant[i] = (uint8_t)v;
-                a.A.days[i] = -2;
-                float sf = a.T.stSusc[fi] ? 1.0f : 0.0f;
-                for (uint32_t u = 0; u < a.T.nVar; ++u) a.A.susc[(size_t)u * n + i] = sf;
-                a.A.infTs[i] = a.timestamp;
-                a.A.infCount[i] = (uint8_t)(a.A.infCount[i] + 1);
-                a.A.infLoc[i] = loc;
-                a.A.worstState[i] = (int8_t)fi;
-                a.A.worstTs[i] = a.timestamp;
-                a.A.statVar[i] = (uint8_t)(a.A.statVar[i] | (1u << v));
-                if (sf == 0.0f) break;
-            }
+    const uint32_t n = a.A.n;
+    const uint32_t base = blockIdx.x * INFECT_TILE + threadIdx.x;
+    uint32_t loc[INFECT_ITEMS];
+#pragma unroll
+    for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
+        uint32_t i = base + k * INFECT_BLOCK;
+        loc[k] = i < n ? __builtin_nontemporal_load(&a.A.location[i]) : NOLOC;
+    }
+    for (uint32_t v = 0; v < a.T.nVar; ++v) {
+        if (!((a.mask >> v) & 1u)) continue;
+        // susceptibility is re-read per variant: an infection by an earlier
+        // variant in this launch has already rewritten it
+        float s[INFECT_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
+            uint32_t i = base + k * INFECT_BLOCK;
+            s[k] = i < n ? a.A.susc[(size_t)v * n + i] : 0.0f;
         }
+        long long e[INFECT_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) e[k] = s[k] > 0.0f ? a.L.sum[(size_t)v * a.L.n + loc[k]] : 0ll;
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
+            if (e[k] != 0) infect_agent(a, base + k * INFECT_BLOCK, v, loc[k], e[k], s[k]);
     }
 }
 
