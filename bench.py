@@ -14,12 +14,13 @@ every agent.  --warmup steps run untimed first (default: day 0), then exactly
 --steps steps are timed between two device synchronisations (default: the
 remaining 27 days).  value = agents * steps / seconds (agent-steps/s).
 
-roofline: the per-step pipeline (mover scan, mover kernel, quarantine-write
-resolve, infection) is the dominant work; its algorithmic bytes per step are
+roofline: the per-step pipeline (movement kernel, quarantine-write resolve,
+infection kernel) is the dominant work; its algorithmic bytes per step are
 22 B/agent + 24 B/location + 4 B/relocated agent, divided by its device time
-from HIP events recorded around every step on the context stream during the
-timed region.  traffic: HBM bytes per step from rocprofv3 FETCH_SIZE /
-WRITE_SIZE when profiles/traffic_synth10M.json exists, else null.
+from HIP events recorded around every movement and infection call on the
+context stream during the timed region.  traffic: HBM bytes per step of the
+same kernels from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+(profiles/traffic_synth10M.json, 10M-agent workload only), else null.
 
 cpu_baseline: the oracle (sequential C++ re-statement of the reference
 per-step pipeline) on a bounded sample of the same workload (1M synthetic
@@ -49,8 +50,24 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-agents", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=144)
+    ap.add_argument("--cpu-steps", type=int, default=1008)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the step kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "repeats": 5}), flush=True)
 
 
 def log(msg):
@@ -78,6 +95,9 @@ def cpu_baseline(a):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (replicas would "
                                    "shard whole populations across GPUs with no collectives)"}))
@@ -119,11 +139,11 @@ def main():
     achieved = step_bytes_total / pipeline_s / 1e9 if pipeline_s > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_synth10M.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("bytes_per_step")
-        except Exception:
-            traffic = None
+    if N == 10_000_000 and os.path.exists(tpath):
+        # HBM bytes per step of the same kernels, from the rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE passes recorded in that file (copy-calibrated)
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_step")
     last = sim.day_stats(sim.num_days - 1)
     cpu = None
     if not a.no_cpu_baseline:

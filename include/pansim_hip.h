@@ -206,6 +206,31 @@ int pansim_progression_day(pansim_ctx* ctx, uint32_t timestamp, uint32_t time_st
 int pansim_immunization_day(pansim_ctx* ctx, uint32_t timestamp, uint32_t time_step, const float protection[60],
                             const int32_t similarity[7], const float similar_multiplier[7]);
 
+/* Immunization::initCategories (immunization.h:229-420): every agent's
+ * vaccination round from its category (health worker, nursery home, 60+,
+ * 18-59 with condition, essential worker, 18-59, 60+ with condition, school
+ * worker, 12-17, 5-11) and one willingness draw.  order[10] =
+ * immunizationOrder, group_level[10] = vaccinationGroupLevel. */
+int pansim_vaccination_init(pansim_ctx* ctx, const int32_t order[10], const float group_level[10]);
+
+/* Agents per 10-year age group (90+ last) for the booster shares
+ * (Immunization::initAgeGroups, immunization.h:421-479). */
+int pansim_age_group_counts(pansim_ctx* ctx, uint32_t out[10]);
+
+/* One booster round of Immunization::update (immunization.h:696-830):
+ * `available` boosters today, split over the age groups by age_group_frac;
+ * eligible agents (first dose > 120 days ago, well, not diagnosed in the last
+ * 120 days, from day 90) are boosted with the group's probability. */
+int pansim_booster_round(pansim_ctx* ctx, uint32_t timestamp, uint32_t time_step, uint32_t round, uint32_t available,
+                         const float age_group_frac[10]);
+
+/* First doses of Immunization::update (immunization.h:836-900): category by
+ * category from *current_category (kept by the caller across days) until the
+ * `available` doses are used; *immunized_today = agents immunized at this
+ * timestamp (boosters included). */
+int pansim_vaccinate(pansim_ctx* ctx, uint32_t timestamp, uint32_t time_step, uint32_t available,
+                     uint32_t* current_category, uint32_t* immunized_today);
+
 /* Simulation::otherDisease (simulation.h:169-423).  probability_mul as
  * computed there from the hospitalisation targets (0 when none are given). */
 int pansim_other_disease_day(pansim_ctx* ctx, uint32_t timestamp, uint32_t time_step, float probability_mul);
@@ -244,7 +269,7 @@ int pansim_upload_schedules(pansim_ctx* ctx, const uint32_t* event_offset, uint3
 /* Downloads a named per-agent or per-location field as uint32 (floats as
  * raw bits): "state", "variant", "days", "location", "stepsUntilMove",
  * "diagnosed", "quarantined", "stayedHome", "noWorkAgent", "lastTest",
- * "susceptible0", AgentStats members, "locQuarantineUntil",
+ * "susceptible0", AgentStats members, "vaccinationRound", "locQuarantineUntil",
  * "locClosedUntil", "locOpen", "locInfectiousness", "locCount".
  * "stepsUntilMove" is reported relative to `timestamp`. */
 int pansim_download_field(pansim_ctx* ctx, const char* name, uint32_t timestamp, uint32_t* out, size_t n);
@@ -272,6 +297,27 @@ int pansim_sync(pansim_ctx* ctx);
 int pansim_device_expf(const float* in, float* out, size_t n);
 int pansim_device_log(const double* in, double* out, size_t n);
 int pansim_device_philox(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4, size_t n);
+
+/* ---- Replica batches --------------------------------------------------
+ * Many contexts (replicas of uneven size, own seed, k, policies) on one
+ * device stepped together: every step is one movement, one quarantine-write
+ * and one infection launch for all replicas.  This has no single counterpart
+ * in the reference, which runs one population per process; each replica's
+ * results are those of stepping its context alone (pansim_movement_step +
+ * pansim_infection_step with the same arguments).  While batched, the
+ * members work on the batch's stream, and their midnight routines are called
+ * on the members as usual; pansim_batch_set_day takes each replica's
+ * movement knobs and infection mask / k / seasonality after its midnight
+ * block, and pansim_batch_step advances every replica by one step at the
+ * shared timestamp.  Plain steps of a member in between are not allowed. */
+typedef struct pansim_batch pansim_batch;
+int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_t n);
+void pansim_batch_destroy(pansim_batch* batch);
+const char* pansim_batch_error(const pansim_batch* batch);
+int pansim_batch_set_day(pansim_batch* batch, const pansim_step_params* params, const uint32_t* variant_masks,
+                         const float* k, const float* seasonality);
+int pansim_batch_step(pansim_batch* batch, uint32_t timestamp);
+int pansim_batch_sync(pansim_batch* batch);
 
 #ifdef __cplusplus
 }

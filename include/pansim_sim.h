@@ -53,6 +53,30 @@ int pansim_sim_header(const pansim_sim* sim, char* out, size_t cap);
 int pansim_sim_sizes(const pansim_sim* sim, uint32_t* n_agents, uint32_t* n_locations, uint32_t* n_variants);
 pansim_ctx* pansim_sim_context(pansim_sim* sim);
 
+/* Replica ensembles: n simulations, each from its own command line (own
+ * population, seed, k, policies; all with the same time step), advanced in
+ * lockstep with one movement and one infection launch per step for all of
+ * them.  Each replica's results are those of running it alone with
+ * pansim_sim_*.  No counterpart in the reference (one population per
+ * process there). */
+typedef struct pansim_ensemble pansim_ensemble;
+int pansim_ensemble_create(pansim_ensemble** out, uint32_t n, const int* argc, const char* const* const* argv);
+const char* pansim_ensemble_create_error(void);
+const char* pansim_ensemble_error(const pansim_ensemble* ens);
+void pansim_ensemble_destroy(pansim_ensemble* ens);
+int pansim_ensemble_steps(pansim_ensemble* ens, uint32_t n);
+int pansim_ensemble_run(pansim_ensemble* ens);
+int pansim_ensemble_sync(pansim_ensemble* ens);
+uint32_t pansim_ensemble_size(const pansim_ensemble* ens);
+uint32_t pansim_ensemble_timestamp(const pansim_ensemble* ens);
+uint32_t pansim_ensemble_steps_per_day(const pansim_ensemble* ens);
+/* replica r's engine context (field downloads through pansim_hip.h) */
+pansim_ctx* pansim_ensemble_context(pansim_ensemble* ens, uint32_t r);
+int pansim_ensemble_num_days(const pansim_ensemble* ens, uint32_t r);
+int pansim_ensemble_day_stats(const pansim_ensemble* ens, uint32_t r, int day, uint32_t* out, size_t cap, size_t* len);
+int pansim_ensemble_sizes(const pansim_ensemble* ens, uint32_t r, uint32_t* n_agents, uint32_t* n_locations,
+                          uint32_t* n_variants);
+
 #ifdef __cplusplus
 }
 #endif

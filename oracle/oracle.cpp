@@ -36,6 +36,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -247,6 +248,12 @@ public:
     std::vector<unsigned> variantCounts;
     unsigned healthcareWorkerCount = 0;
     unsigned immunizedToday = 0;
+    // vaccination rollout (immunization.h:24-61)
+    std::vector<uint8_t> immRound;  // category round of every agent, 255 = not willing
+    unsigned currentCategory = 0;
+    bool ageGroupsReady = false;
+    std::vector<std::array<unsigned, 10>> ageGroupSize;
+    std::vector<std::array<float, 10>> ageGroupFrac;
     std::vector<std::vector<unsigned>> dayStats;  // every statistics vector produced
     std::vector<unsigned> lastStats;
     // closures
@@ -273,7 +280,12 @@ public:
 
     // reference routines
     unsigned findLoc(unsigned agent, unsigned locType, int retry, bool useStates, Rng* rng);
-    void quarantineAgent(unsigned i, unsigned until, std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng);
+    void quarantineAgent(unsigned i, unsigned until);
+    void quarantineLocations(unsigned i, unsigned until, unsigned previousQuarantineUntil,
+                             std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng);
+    void doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng);
+    bool qLoc = false;  // quarantineLocations pending for the agent being moved
+    unsigned qLocUntil = 0, qLocPrev = 0;
     void doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair<unsigned, unsigned>>& writes);
     void movement();
     void rebuildLocationSums();
@@ -283,6 +295,10 @@ public:
     void performTests();
     void updateAgents();
     void immunizationUpdate();
+    bool inVaccinationCategory(unsigned i, unsigned g) const;
+    void initCategories();
+    void initAgeGroups();
+    void vaccinationRollout();
     void otherDisease();
     std::vector<unsigned> refreshStatistics();
     void closureMidnight(std::vector<unsigned>& st);
@@ -509,6 +525,7 @@ Sim::Sim(const Scenario& s) : sc(s) {
         if (!cur.empty()) header.push_back(cur);
     }
     initClosures();
+    initCategories();
 }
 
 // DynamicPPState(name, progressionID): daysBeforeNextState = calculateJustDays(state, 0)
@@ -564,8 +581,11 @@ unsigned Sim::findLoc(unsigned agent, unsigned lt, int retry, bool useStates, Rn
     return UMAX;
 }
 
-// movementPolicies.h:171-245.  Location writes are collected in `writes`.
-void Sim::quarantineAgent(unsigned i, unsigned until, std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng) {
+// movementPolicies.h:171-245 without its location writes.  An agent is
+// quarantined at most once per movement routine; for a diagnosed one the
+// location writes of quarantineLocations follow after the routine (so its
+// classroom draw is the routine's last draw).
+void Sim::quarantineAgent(unsigned i, unsigned until) {
     if (quarantinePolicy == 0) return;
     if (stats[i].diagnosedTimestamp > 0 && !isInfected(i) && quarantineImmuneActive == false) return;
     if (!diagnosed[i] && stats[i].immunizationTimestamp > 0 && quarantineImmuneActive == false) return;
@@ -575,17 +595,27 @@ void Sim::quarantineAgent(unsigned i, unsigned until, std::vector<std::pair<unsi
     stats[i].quarantinedUntilTimestamp = until;
     stats[i].daysInQuarantine += (until - timestamp) / (24 * 60 / dt);
     if (diagnosed[i]) {
-        unsigned myHome = findLoc(i, homeT, 0, false, nullptr);
-        if (quarantinePolicy > 1 && myHome != UMAX) writes.emplace_back(myHome, until);  // the reference writes out of bounds
-        if (quarantinePolicy > 2 && previousQuarantineUntil < timestamp) {
-            unsigned classroom = findLoc(i, classroomT, 0, false, &rng);
-            unsigned work = findLoc(i, workT, 0, false, nullptr);
-            unsigned toClose[2] = {classroom, work};
-            for (unsigned loc : toClose) {
-                if (loc != UMAX && (locType[loc] == workT || locType[loc] == classroomT) && quarantineUntil[loc] < timestamp &&
-                    locOpen[loc] && closedUntil[loc] < timestamp)
-                    writes.emplace_back(loc, until);
-            }
+        qLoc = true;
+        qLocUntil = until;
+        qLocPrev = previousQuarantineUntil;
+    }
+}
+
+// movementPolicies.h:202-241: the home, and with quarantinePolicy > 2 the
+// classroom and workplace, of a quarantined diagnosed agent.  Location writes
+// are collected in `writes`.
+void Sim::quarantineLocations(unsigned i, unsigned until, unsigned previousQuarantineUntil,
+                              std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng) {
+    unsigned myHome = findLoc(i, homeT, 0, false, nullptr);
+    if (quarantinePolicy > 1 && myHome != UMAX) writes.emplace_back(myHome, until);  // the reference writes out of bounds
+    if (quarantinePolicy > 2 && previousQuarantineUntil < timestamp) {
+        unsigned classroom = findLoc(i, classroomT, 0, false, &rng);
+        unsigned work = findLoc(i, workT, 0, false, nullptr);
+        unsigned toClose[2] = {classroom, work};
+        for (unsigned loc : toClose) {
+            if (loc != UMAX && (locType[loc] == workT || locType[loc] == classroomT) && quarantineUntil[loc] < timestamp &&
+                locOpen[loc] && closedUntil[loc] < timestamp)
+                writes.emplace_back(loc, until);
         }
     }
 }
@@ -596,13 +626,23 @@ static inline float fmin_ref(float a, float b) { return a < b ? a : b; }
 
 // movementPolicies.h:258-1041
 void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair<unsigned, unsigned>>& writes) {
-    unsigned agentType = types[i];
     Rng rng(seed, i, timestamp, PHASE_MOVE);
+    qLoc = false;
+    doMovementRoutine(i, day, simTime, rng);
+    if (qLoc) quarantineLocations(i, qLocUntil, qLocPrev, writes, rng);
+}
+
+void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
+    unsigned agentType = types[i];
     unsigned& sum = stepsUntilMove[i];
     Stats& st = stats[i];
     const unsigned stepsUntilMidnight = simTime.stepsUntilMidnight(dt);
     const unsigned qLen = quarantineLength * 24 * 60 / dt;
-    auto locQ = [&](unsigned loc) { return quarantineUntil[loc]; };
+    // a missing location (UMAX: no entry of the requested type) reads as
+    // closed, never quarantined and of no type; the GPU path does the same
+    auto locQ = [&](unsigned loc) { return loc < L ? quarantineUntil[loc] : 0u; };
+    auto openAt = [&](unsigned loc) { return loc < L ? (unsigned)locOpen[loc] : 0u; };
+    auto closedAt = [&](unsigned loc) { return loc < L ? closedUntil[loc] : 0u; };
 
     if (enableCurfew && curfewBegin == simTime.minutes() / dt) {
         uint8_t w = wbOf(i);
@@ -611,9 +651,9 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
         if (!deadOrHosp && !hospNonCovid) {
             if (agentType + 1 == 7) {
                 unsigned workplace = findLoc(i, workT, 0, false, nullptr);
-                if (location[i] != workplace || locOpen[workplace] == 0) {
+                if (location[i] != workplace || openAt(workplace) == 0) {
                     location[i] = findLoc(i, homeT, 0, false, nullptr);
-                    if (locOpen[workplace] == 0) {
+                    if (openAt(workplace) == 0) {
                         sum = stepsUntilMidnight;
                         return;
                     }
@@ -644,7 +684,7 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
             diagnosed[i] = 1;
             st.diagnosedTimestamp = timestamp;
             if (stepsUntilMidnight == 24 * 60 / dt) st.diagnosedTimestamp++;
-            quarantineAgent(i, timestamp + qLen, writes, rng);
+            quarantineAgent(i, timestamp + qLen);
         }
     };
 
@@ -661,11 +701,11 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
         return;
     }
     auto quarantinableHere = [&](unsigned loc) {
-        unsigned t = locType[loc];
+        unsigned t = loc < L ? locType[loc] : 255u;
         return t == homeT || t == schoolT || t == classroomT || t == workT;
     };
     if (quarantinePolicy > 1 && timestamp < locQ(location[i]) && quarantinableHere(location[i])) {
-        if (quarantined[i] == 0) quarantineAgent(i, locQ(location[i]), writes, rng);
+        if (quarantined[i] == 0) quarantineAgent(i, locQ(location[i]));
         if (quarantined[i]) {
             sum = umin(locQ(location[i]) - timestamp - 1, stepsUntilMidnight);
             unsigned homeLocation = findLoc(i, homeT, 0, false, nullptr);
@@ -677,7 +717,7 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
     if ((quarantinePolicy > 0 &&
          (diagnosed[i] || (st.diagnosedTimestamp > 0 && (timestamp - st.diagnosedTimestamp) < qLen))) ||
         (quarantinePolicy > 0 && quarantined[i])) {
-        if (quarantinePolicy > 0 && !quarantined[i]) quarantineAgent(i, timestamp + qLen, writes, rng);
+        if (quarantinePolicy > 0 && !quarantined[i]) quarantineAgent(i, timestamp + qLen);
         if (quarantined[i]) {
             location[i] = findLoc(i, homeT, 0, false, nullptr);
             sum = stepsUntilMidnight;
@@ -736,7 +776,7 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
         bool schoolAndTooOld =
             (newLocationType == schoolT || newLocationType == classroomT) && meta[i].age >= schoolAgeRestriction;
         auto closedHere = [&](unsigned loc, unsigned lt) {
-            return (locOpen[loc] == 0 || closedUntil[loc] > timestamp) && lt != workT;
+            return (openAt(loc) == 0 || closedAt(loc) > timestamp) && lt != workT;
         };
         if (schoolAndTooOld || closedHere(newLocation, newLocationType) ||
             (lockdownNonvaccActive && st.immunizationTimestamp == 0)) {
@@ -787,7 +827,7 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
         if (agentType + 1 == 7 && enableCurfew &&
             (curfewBegin <= simTime.minutes() / dt || curfewEnd > simTime.minutes() / dt)) {
             bool workButClosed =
-                newLocationType == workT && (locOpen[newLocation] == 0 || closedUntil[newLocation] > timestamp);
+                newLocationType == workT && (openAt(newLocation) == 0 || closedAt(newLocation) > timestamp);
             if (workButClosed || (newLocationType != workT && newLocationType != homeT)) location[i] = agentHome;
         }
     }
@@ -811,7 +851,7 @@ void Sim::doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair
     }
     // arrived somewhere under quarantine
     if (quarantinePolicy > 1 && timestamp < locQ(location[i]) && quarantinableHere(location[i])) {
-        if (!quarantined[i]) quarantineAgent(i, locQ(location[i]), writes, rng);
+        if (!quarantined[i]) quarantineAgent(i, locQ(location[i]));
         if (quarantined[i]) {
             sum = umin(locQ(location[i]) - timestamp - 1, stepsUntilMidnight);
             unsigned homeLocation = findLoc(i, homeT, 0, false, nullptr);
@@ -831,7 +871,9 @@ void Sim::movement() {
     std::vector<std::pair<unsigned, unsigned>> all, w;
     for (unsigned i = 0; i < N; ++i) {
         w.clear();
+        unsigned old = location[i];
         doMovement(i, day, simTime, w);
+        if (location[i] == UMAX) location[i] = old;  // no entry of the requested type: stay
         all.insert(all.end(), w.begin(), w.end());
     }
     for (auto& x : all) quarantineUntil[x.first] = x.second;  // ascending agent order: last writer wins
@@ -1250,16 +1292,163 @@ void Sim::immunizationUpdate() {
         for (unsigned v = 0; v < nVar; ++v)
             if (p.state == 0) p.susceptible[v] = fmin_ref(1.0f, susc[v]);
     }
-    // rollout guard
-    unsigned simDay = timestamp / stepsPerDay;
-    auto boosters = ps::split_int(sc.opt.boosterStart, ',');
-    bool vaccActive = simDay >= sc.opt.immunizationStart;
-    bool boostActive = false;
-    for (int b : boosters) boostActive = boostActive || (int)simDay >= b;
-    if (vaccActive || boostActive)
-        throw ps::InitError(ps::ERR_UNSUPPORTED,
-                            "vaccination/booster rollout (day >= immunizationStart/boosterStart) is not implemented");
+    vaccinationRollout();
+}
+
+// The ten vaccination categories of initCategories (immunization.h:229-339),
+// indexed as the reference's pass loop indexes them (groupIdx).  Location-type
+// ids 2, 3, 4, 12, 14, 22 are literals in the reference.
+bool Sim::inVaccinationCategory(unsigned i, unsigned g) const {
+    const unsigned age = meta[i].age, pre = meta[i].preCondIdx;
+    auto anyEntry = [&](auto pred) {
+        for (uint32_t e = locOffset[i]; e < locOffset[i + 1]; ++e)
+            if (pred((unsigned)possibleTypes[e], possibleLocations[e])) return true;
+        return false;
+    };
+    switch (g) {
+        case 0: return anyEntry([&](unsigned t, unsigned l) { return t == 4 && (locType[l] == 12 || locType[l] == 14); });
+        case 1: return anyEntry([&](unsigned t, unsigned l) { return (t == 4 || t == 2) && locType[l] == 22; });
+        case 2: return age >= 60;
+        case 3: return pre > 0 && age >= 18 && age < 60;
+        case 4: return anyEntry([&](unsigned t, unsigned l) { return t == 4 && essential[l] == 1; });
+        case 5: return age > 17 && age < 60;
+        case 6: return pre > 0 && age >= 60;
+        case 7: return anyEntry([&](unsigned t, unsigned l) { return t == 4 && locType[l] == 3; });
+        case 8: return age >= 12 && age < 18;
+        case 9: return age >= 5 && age < 12;
+    }
+    return false;
+}
+
+// immunization.h:229-420: every agent's category round (its category's
+// position in immunizationOrder) or 255 when it declines.  Groups are visited
+// in ascending order value, ties by ascending index; an agent is decided by
+// the first visited group it belongs to (one draw).  The level of group g is
+// vaccinationGroupLevel[{0,1,3,4,5,7,2,6,8,9}[g]], as the reference's lambdas
+// capture it.
+void Sim::initCategories() {
+    static const int levelOf[10] = {0, 1, 3, 4, 5, 7, 2, 6, 8, 9};
+    const auto& v = sc.vacc;
+    immRound.assign(N, 0);
+    std::vector<unsigned> passes;
+    for (int o = 1; o <= 10; ++o)
+        for (unsigned g = 0; g < 10; ++g)
+            if (v.order[g] == o) passes.push_back(g);
+    for (unsigned i = 0; i < N; ++i) {
+        Rng rng(seed, i, 0, PHASE_IMM_CATEGORY);
+        for (unsigned g : passes) {
+            if (!inVaccinationCategory(i, g)) continue;
+            immRound[i] = rng.unit() < (double)v.groupLevel[levelOf[g]] ? (uint8_t)v.order[g] : (uint8_t)255;
+            break;
+        }
+    }
+}
+
+static inline int ageGroupOf(unsigned age) { return age < 90 ? (int)(age / 10) : (age < 200 ? 9 : -1); }
+
+// immunization.h:421-479
+void Sim::initAgeGroups() {
+    ageGroupsReady = true;
+    const auto& v = sc.vacc;
+    const size_t rounds = v.boosterStart.size();
+    ageGroupSize.assign(rounds, {});
+    ageGroupFrac.assign(rounds, {});
+    std::array<unsigned, 10> ages{};
+    for (unsigned i = 0; i < N; ++i) {
+        int g = ageGroupOf(meta[i].age);
+        if (g >= 0) ages[g]++;
+    }
+    for (size_t r = 0; r < rounds; ++r) {
+        unsigned sum = 0;
+        for (int g = 0; g < 10; ++g) {
+            ageGroupSize[r][g] = (unsigned)((float)ages[g] * v.boosterAgePercent[r][g]);
+            sum += ageGroupSize[r][g];
+        }
+        for (int g = 0; g < 10; ++g) ageGroupFrac[r][g] = (float)ageGroupSize[r][g] / (float)sum;
+    }
+}
+
+// immunization.h:696-900: booster rounds, then first doses category by
+// category, then immunizedToday = agents immunized at this timestamp.
+void Sim::vaccinationRollout() {
+    const auto& v = sc.vacc;
+    const unsigned spd = stepsPerDay;
+    const unsigned ts = timestamp == 0 ? 1u : timestamp;
+    const unsigned day = timestamp / spd;
+    if (!ageGroupsReady) initAgeGroups();
+    auto isW = [&](unsigned i) { return wbOf(i) == ps::WB_W; };
+    auto boosterEligible = [&](unsigned i) {
+        const Stats& s = stats[i];
+        return s.immunizationTimestamp > 0 && s.immunizationCount >= 1 && (ts - s.immunizationTimestamp) / spd > 4 * 30 &&
+               isW(i) && (ts >= spd * 3 * 30 && s.diagnosedTimestamp < ts - spd * 4 * 30);
+    };
+    for (size_t r = 0; r < v.boosterStart.size(); ++r) {
+        unsigned available = 0;
+        if ((int)day >= v.boosterStart[r]) {
+            if (v.boosterPerWeek[r].empty()) available = v.dailyBoosters;
+            else {
+                unsigned week = (day - (unsigned)v.boosterStart[r]) / 7;
+                const auto& w = v.boosterPerWeek[r];
+                available = (unsigned)(w[week >= w.size() ? w.size() - 1 : week] / 7.0);
+            }
+        }
+        if (!available) continue;
+        std::array<unsigned, 10> counts{};
+        for (unsigned i = 0; i < N; ++i) {
+            int g = ageGroupOf(meta[i].age);
+            if (g >= 0 && boosterEligible(i)) counts[g]++;
+        }
+        for (int g = 0; g < 10; ++g) {
+            unsigned availG = (unsigned)((float)available * ageGroupFrac[r][g]);
+            float prob = counts[g] < availG ? 1.0f : (float)availG / (float)counts[g];
+            for (unsigned i = 0; i < N; ++i) {
+                if (ageGroupOf(meta[i].age) != g || !boosterEligible(i)) continue;
+                Rng rng(seed, i, timestamp, PHASE_VACCINATION);
+                rng.seek((uint32_t)r);
+                if (prob == 1.0f || rng.unit() < (double)prob) {
+                    stats[i].immunizationTimestamp = ts;
+                    stats[i].immunizationCount++;
+                }
+            }
+        }
+    }
+    unsigned available = 0;
+    if (day >= v.startDay) {
+        if (v.vaccPerWeek.empty()) available = v.dailyDoses;
+        else {
+            unsigned week = (day - v.startDay) / 7;
+            available = (unsigned)(v.vaccPerWeek[week >= v.vaccPerWeek.size() ? v.vaccPerWeek.size() - 1 : week] / 7.0);
+        }
+    }
+    auto vaccEligible = [&](unsigned i) {
+        const Stats& s = stats[i];
+        return s.immunizationCount == 0 && isW(i) &&
+               ((ts < spd * 3 * 30 && s.diagnosedTimestamp == 0) ||
+                (ts >= spd * 3 * 30 && s.diagnosedTimestamp < ts - spd * 3 * 30));
+    };
+    while (available > 0 && currentCategory < 10) {
+        unsigned count = 0;
+        while (count == 0 && currentCategory <= 10) {
+            const unsigned cc = currentCategory + 1;  // rounds start at 1; 11 counts every round
+            count = 0;
+            for (unsigned i = 0; i < N; ++i) count += (immRound[i] == cc || cc == 11) && vaccEligible(i);
+            if (count == 0) currentCategory++;
+        }
+        float prob = count < available ? 1.0f : (float)available / (float)count;
+        const unsigned cc = currentCategory + 1;
+        for (unsigned i = 0; i < N; ++i) {
+            if (immRound[i] != cc || !vaccEligible(i)) continue;
+            Rng rng(seed, i, timestamp, PHASE_VACCINATION);
+            rng.seek(32u + currentCategory);
+            if (prob == 1.0f || rng.unit() < (double)prob) {
+                stats[i].immunizationTimestamp = ts;
+                stats[i].immunizationCount = 1;
+            }
+        }
+        available = count < available ? available - count : 0;
+    }
     immunizedToday = 0;
+    for (unsigned i = 0; i < N; ++i) immunizedToday += stats[i].immunizationTimestamp == ts;
 }
 
 // simulation.h:169-423
@@ -1872,6 +2061,9 @@ int oracle_field(void* h, const char* name, unsigned* out) {
     else if (n == "hospitalizedTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].hospitalizedTimestamp;
     else if (n == "hospitalizedUntilTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].hospitalizedUntilTimestamp;
     else if (n == "recoveredTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].recoveredTimestamp;
+    else if (n == "immunizationTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].immunizationTimestamp;
+    else if (n == "immunizationCount") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].immunizationCount;
+    else if (n == "vaccinationRound") for (unsigned i = 0; i < N; ++i) out[i] = s->immRound[i];
     else if (n == "infectedCount") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].infectedCount;
     else if (n == "statsVariant") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].variant;
     else if (n == "worstState") for (unsigned i = 0; i < N; ++i) out[i] = (unsigned)(int)s->stats[i].worstState;

@@ -63,6 +63,7 @@ public:
         return (uint32_t)(((uint64_t)w[2] * max) >> 32);
     }
     uint32_t draws() const { return n_; }
+    void seek(uint32_t n) { n_ = n; }  // continue the sequence at draw n
 
 private:
     uint64_t seed_;
@@ -72,6 +73,7 @@ private:
 
 enum : uint32_t {
     PHASE_INIT_DAYS = 1,
+    PHASE_IMM_CATEGORY = 2,
     PHASE_MOVE = 3,
     PHASE_PLAN_SCHOOLWORK = 4,
     PHASE_TESTING = 5,
@@ -79,6 +81,7 @@ enum : uint32_t {
     PHASE_OTHER_DISEASE = 7,
     PHASE_INFECT = 8,
     PHASE_MUTATION = 16,
+    PHASE_VACCINATION = 17,
     PHASE_TEST_FLAG = 18,
     PHASE_PLAN_UNDERAGE = 19
 };

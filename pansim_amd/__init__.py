@@ -33,7 +33,7 @@ AGENT_FIELDS = [
     "noWorkAgent", "lastTest", "susceptible0", "infectedTimestamp", "infectedLocation", "worstStateTimestamp",
     "worstStateEndTimestamp", "diagnosedTimestamp", "quarantinedTimestamp", "quarantinedUntilTimestamp",
     "daysInQuarantine", "hospitalizedTimestamp", "hospitalizedUntilTimestamp", "recoveredTimestamp",
-    "infectedCount", "statsVariant", "worstState",
+    "infectedCount", "statsVariant", "worstState", "immunizationTimestamp", "immunizationCount", "vaccinationRound",
 ]
 LOCATION_FIELDS = ["locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness", "locCount"]
 
@@ -87,6 +87,27 @@ def lib() -> ctypes.CDLL:
     L.pansim_device_expf.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]
     L.pansim_device_log.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
     L.pansim_device_philox.argtypes = [c_u32p, ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_size_t]
+    L.pansim_ensemble_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p))]
+    L.pansim_ensemble_create_error.restype = ctypes.c_char_p
+    L.pansim_ensemble_error.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_error.restype = ctypes.c_char_p
+    L.pansim_ensemble_destroy.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_ensemble_run.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_sync.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_size.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_size.restype = ctypes.c_uint32
+    L.pansim_ensemble_timestamp.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_timestamp.restype = ctypes.c_uint32
+    L.pansim_ensemble_steps_per_day.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_steps_per_day.restype = ctypes.c_uint32
+    L.pansim_ensemble_context.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_ensemble_context.restype = ctypes.c_void_p
+    L.pansim_ensemble_num_days.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_ensemble_day_stats.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, c_u32p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)]
+    L.pansim_ensemble_sizes.argtypes = [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_u32p, c_u32p]
     _lib = L
     return L
 
@@ -220,6 +241,94 @@ class Simulation:
         if rc != 0:
             raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
         return out
+
+
+class Ensemble:
+    """Replicas (each from its own command line: population, seed, k,
+    policies) advanced in lockstep with one movement and one infection launch
+    per step for all of them.  replica(r) is a read-only view with the
+    Simulation accessors (fields, location sums, per-day statistics)."""
+
+    def __init__(self, replica_args: Sequence[Sequence[str]]):
+        L = lib()
+        self._args = [with_data_dir(a) for a in replica_args]
+        n = len(self._args)
+        argcs = (ctypes.c_int * n)(*[len(a) for a in self._args])
+        self._argvs = [_argv(a) for a in self._args]
+        argvs = (ctypes.POINTER(ctypes.c_char_p) * n)(*[ctypes.cast(a, ctypes.POINTER(ctypes.c_char_p)) for a in self._argvs])
+        h = ctypes.c_void_p()
+        rc = L.pansim_ensemble_create(ctypes.byref(h), n, argcs, argvs)
+        if rc != 0:
+            raise PanSimError(rc, L.pansim_ensemble_create_error().decode())
+        self._h = h
+        self.size = n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pansim_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_ensemble_error(self._h).decode())
+
+    def steps(self, n: int):
+        self._check(lib().pansim_ensemble_steps(self._h, n))
+
+    def run(self):
+        self._check(lib().pansim_ensemble_run(self._h))
+
+    def sync(self):
+        self._check(lib().pansim_ensemble_sync(self._h))
+
+    @property
+    def timestamp(self) -> int:
+        return lib().pansim_ensemble_timestamp(self._h)
+
+    def replica(self, r: int) -> "EnsembleReplica":
+        return EnsembleReplica(self, r)
+
+
+class EnsembleReplica:
+    """Read-only view of one replica of an Ensemble."""
+
+    def __init__(self, ens: Ensemble, r: int):
+        if not 0 <= r < ens.size:
+            raise IndexError(r)
+        self._e, self._r = ens, r
+        na, nl, nv = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        lib().pansim_ensemble_sizes(ens._h, r, ctypes.byref(na), ctypes.byref(nl), ctypes.byref(nv))
+        self.n_agents, self.n_locations, self.n_variants = na.value, nl.value, nv.value
+        self.steps_per_day = lib().pansim_ensemble_steps_per_day(ens._h)
+
+    @property
+    def ctx(self):
+        return lib().pansim_ensemble_context(self._e._h, self._r)
+
+    @property
+    def timestamp(self) -> int:
+        return self._e.timestamp
+
+    @property
+    def num_days(self) -> int:
+        return lib().pansim_ensemble_num_days(self._e._h, self._r)
+
+    def day_stats(self, d: int) -> List[int]:
+        out = (ctypes.c_uint32 * 512)()
+        n = ctypes.c_size_t()
+        rc = lib().pansim_ensemble_day_stats(self._e._h, self._r, d, out, 512, ctypes.byref(n))
+        if rc != 0:
+            raise PanSimError(rc, "day_stats")
+        return list(out[: n.value])
+
+    field = Simulation.field
+    location_sums = Simulation.location_sums
 
 
 class SimulatorInterface:

@@ -50,8 +50,8 @@ __global__ void k_init_agents(Agents A, Tables T, uint64_t seed, uint32_t replic
                               uint32_t qlenDays) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    uint8_t st = A.state[i];
-    A.variant[i] = 0;
+    uint8_t st = A.rec[i].state;
+    A.rec[i].variant = 0;
     update_meta(A, T, i, st);
     for (uint32_t v = 0; v < T.nVar; ++v) A.slot6[(size_t)v * A.n + i] = A.scal[i];
     Stream rs(seed, i, 0, PH_INIT_DAYS, replica);
@@ -64,36 +64,37 @@ __global__ void k_init_agents(Agents A, Tables T, uint64_t seed, uint32_t replic
     A.worstEndTs[i] = 0;
     A.infLoc[i] = 0;
     A.recTs[i] = NEVER;
-    A.hospTs[i] = 0;
-    A.hospUntil[i] = 0;
+    A.rec[i].hospTs = 0;
+    A.rec[i].hospUntil = 0;
     A.immTs[i] = 0;
     A.immCount[i] = 0;
     A.statVar[i] = 0;
     A.lastTest[i] = NEVER;
     A.wake[i] = 0;
-    uint8_t fl = A.flags[i];  // diagnosed / hcw preset by the host upload
+    uint8_t fl = A.rec[i].flags;  // diagnosed / hcw preset by the host upload
     bool diag = fl & F_DIAGNOSED;
-    A.diagTs[i] = diag ? 1u : 0u;
+    A.rec[i].diagTs = diag ? 1u : 0u;
     if (diag && qp > 0) {
         A.quarTs[i] = 0;
-        A.quarUntil[i] = qlenSteps;
+        A.rec[i].quarUntil = qlenSteps;
         A.daysInQ[i] = qlenDays;
         fl |= F_QUARANTINED;
     } else {
         A.quarTs[i] = 0;
-        A.quarUntil[i] = 0;
+        A.rec[i].quarUntil = 0;
         A.daysInQ[i] = 0;
     }
     fl |= F_STAYED_HOME;
-    A.flags[i] = fl;
+    A.rec[i].flags = fl;
 }
 
 // ------------------------------------------------------------ occupancy
 __global__ void k_zero_locations(Locs L, uint32_t nVar) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
-        L.count[l] = 0u;
-        for (uint32_t v = 0; v < nVar; ++v) L.sum[(size_t)v * L.n + l] = 0;
+        L.dyn[l].count = 0u;
+        L.dyn[l].sum0 = 0;
+        for (uint32_t v = 1; v < nVar; ++v) L.sumX[(size_t)(v - 1) * L.n + l] = 0;
     }
 }
 
@@ -101,10 +102,10 @@ __global__ void k_scatter_locations(Agents A, Locs L, Tables T) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     uint32_t loc = A.location[i];
-    atomicAdd(&L.count[loc], 1u);
-    float x = T.stInf[A.state[i]] * T.varMul[A.variant[i]];
-    if (x != 0.0f)
-        atomicAdd((unsigned long long*)&L.sum[(size_t)A.variant[i] * L.n + loc], (unsigned long long)fixed32(x));
+    atomicAdd(&L.dyn[loc].count, 1u);
+    const AgentRec r = A.rec[i];
+    float x = T.stInf[r.state] * T.varMul[r.variant];
+    if (x != 0.0f) atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)fixed32(x));
 }
 
 __global__ void k_count_variants(Agents A, Tables T, unsigned long long* out) {
@@ -112,7 +113,7 @@ __global__ void k_count_variants(Agents A, Tables T, unsigned long long* out) {
     if (threadIdx.x < MAXV) sh[threadIdx.x] = 0;
     __syncthreads();
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < A.n && T.stInfected[A.state[i]]) atomicAdd(&sh[A.variant[i]], 1u);
+    if (i < A.n && T.stInfected[A.rec[i].state]) atomicAdd(&sh[A.rec[i].variant], 1u);
     __syncthreads();
     if (threadIdx.x < T.nVar && sh[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
 }
@@ -130,7 +131,7 @@ struct TestArgs {
 
 __global__ void k_clear_flags(Locs L) {
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) L.flag[l] = 0;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) L.rec[l].flag = 0;
 }
 
 // DetailedTestingOps::flagLocations (testingPolicies.h:59-148)
@@ -138,23 +139,23 @@ __global__ void k_flag_locations(TestArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.A.n) return;
     const uint32_t day = 24u * 60u / a.dt;
-    if (!(a.A.diagTs[i] > a.ts - day)) return;
+    if (!(a.A.rec[i].diagTs > a.ts - day)) return;
     const pansim_location_types& lt = a.T.lt;
     const uint32_t ql = a.qlen * 24u * 60u / a.dt;
     auto ok = [&](uint32_t l) {
-        uint32_t q = a.L.quarUntil[l];
+        uint32_t q = a.L.rec[l].quarUntil;
         return q == 0 || (q != 0 && (q - ql >= a.ts - day || q < a.ts));
     };
     uint32_t home = find_loc(a.A, a.L, lt, i, lt.home, 0, nullptr);
-    if (home != NOLOC) a.L.flag[home] = 1;
+    if (home != NOLOC) a.L.rec[home].flag = 1;
     uint32_t work = find_loc(a.A, a.L, lt, i, lt.work, 0, nullptr);
-    if (work != NOLOC && ok(work)) a.L.flag[work] = 1;
+    if (work != NOLOC && ok(work)) a.L.rec[work].flag = 1;
     uint32_t school = find_loc(a.A, a.L, lt, i, lt.school, 0, nullptr);
     if (school != NOLOC && ok(school)) {
-        a.L.flag[school] = 1;
+        a.L.rec[school].flag = 1;
         Stream rs(a.seed, i, a.ts, PH_TEST_FLAG, a.replica);
         uint32_t cls = find_loc(a.A, a.L, lt, i, lt.classroom, 0, &rs);
-        if (cls != NOLOC && ok(cls)) a.L.flag[cls] = 1;
+        if (cls != NOLOC && ok(cls)) a.L.rec[cls].flag = 1;
     }
 }
 
@@ -169,19 +170,19 @@ __global__ void k_do_testing(TestArgs a) {
         const uint32_t day = 24u * 60u / a.dt;
         const uint32_t ts = a.ts;
         const pansim_location_types& lt = a.T.lt;
-        uint8_t fl = A.flags[i];
+        uint8_t fl = A.rec[i].flags;
         bool skip = ts > a.delay * day && A.lastTest[i] != NEVER && A.lastTest[i] > ts - a.delay * day;
-        uint8_t st = A.state[i];
+        uint8_t st = A.rec[i].state;
         if (!skip && !(a.T.stWB[st] == 4 || (fl & F_DIAGNOSED))) {
             Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
             uint32_t home = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
-            bool homeFlag = home != NOLOC ? a.L.flag[home] != 0 : false;
+            bool homeFlag = home != NOLOC ? a.L.rec[home].flag != 0 : false;
             uint32_t work = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
-            bool workFlag = work != NOLOC ? a.L.flag[work] != 0 : false;
+            bool workFlag = work != NOLOC ? a.L.rec[work].flag != 0 : false;
             uint32_t school = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
             bool schoolFlag = false, classFlag = false;
             if (school != NOLOC) {
-                schoolFlag = a.L.flag[school] != 0;
+                schoolFlag = a.L.rec[school].flag != 0;
                 uint32_t cls = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
                 if (cls != NOLOC) classFlag = true;
             }
@@ -190,27 +191,27 @@ __global__ void k_do_testing(TestArgs a) {
             prob += (double)workFlag * a.pWork;
             prob += (double)schoolFlag * a.pSchool;
             prob += (double)classFlag * 3.0 * a.pSchool;
-            if (work != NOLOC && (a.L.type[work] == lt.doctor || a.L.type[work] == lt.hospital)) prob += a.pHospital;
-            if (work != NOLOC && a.L.type[work] == lt.nursery_home) prob += a.pNursery;
-            if (A.hospTs[i] <= ts && A.hospUntil[i] > ts) prob += a.pHospital;
+            if (work != NOLOC && (a.L.rec[work].type == lt.doctor || a.L.rec[work].type == lt.hospital)) prob += a.pHospital;
+            if (work != NOLOC && a.L.rec[work].type == lt.nursery_home) prob += a.pNursery;
+            if (A.rec[i].hospTs <= ts && A.rec[i].hospUntil > ts) prob += a.pHospital;
             if (prob > 1.0 || rs.real(1.0) < prob) {
                 A.lastTest[i] = ts;
                 if (a.T.stInfected[st]) {
                     float pr = a.usePCR ? a.T.accPCR[st] : a.T.accAnt[st];
                     if ((double)pr > rs.real(1.0)) {
-                        A.flags[i] = (uint8_t)(fl | F_DIAGNOSED);
-                        A.diagTs[i] = ts;
+                        A.rec[i].flags = (uint8_t)(fl | F_DIAGNOSED);
+                        A.rec[i].diagTs = ts;
                     }
                 } else {
-                    if (A.quarUntil[i] > ts && (home != NOLOC && a.L.quarUntil[home] < ts)) {
-                        if (A.daysInQ[i] > 0) A.daysInQ[i] -= (A.quarUntil[i] - ts) / day;
-                        A.quarUntil[i] = ts;
+                    if (A.rec[i].quarUntil > ts && (home != NOLOC && a.L.rec[home].quarUntil < ts)) {
+                        if (A.daysInQ[i] > 0) A.daysInQ[i] -= (A.rec[i].quarUntil - ts) / day;
+                        A.rec[i].quarUntil = ts;
                     }
                 }
             }
         }
         if (A.lastTest[i] == ts) atomicAdd(&sh[0], 1u);
-        uint32_t d = A.diagTs[i];
+        uint32_t d = A.rec[i].diagTs;
         if (d == ts) atomicAdd(&sh[1], 1u);
         if (d < ts && d > ts - day) atomicAdd(&sh[2], 1u);
     }
@@ -230,8 +231,8 @@ __global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replic
         return;
     }
     if (days != -2 && days != 0) return;
-    uint8_t st = A.state[i];
-    uint8_t v = A.variant[i];
+    uint8_t st = A.rec[i].state;
+    uint8_t v = A.rec[i].variant;
     uint32_t prog = A.prog[i];
     float progS;
     if (st == 7) progS = T.deathScaling[v] / scaling_symptoms(A, i, v, st);
@@ -284,7 +285,7 @@ __global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replic
         }
         int nd = calc_days(T, prog, next, v, rs);
         st = (uint8_t)next;
-        A.state[i] = st;
+        A.rec[i].state = st;
         update_meta(A, T, i, st);
         days = (int16_t)nd;
         if (v >= 3 && st < 8) {
@@ -299,7 +300,7 @@ __global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replic
         } else if (old != st) {
             if (wasInf && !T.stInfected[st]) {
                 A.recTs[i] = ts;
-                A.flags[i] = (uint8_t)(A.flags[i] & ~F_DIAGNOSED);
+                A.rec[i].flags = (uint8_t)(A.rec[i].flags & ~F_DIAGNOSED);
             }
         }
     }
@@ -323,7 +324,7 @@ __global__ void k_immunization(ImmArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    if (A.state[i] != 0) return;
+    if (A.rec[i].state != 0) return;
     const uint32_t nVar = a.T.nVar;
     const uint32_t ts = a.ts == 0 ? 1u : a.ts;
     const uint32_t week = 7u * 24u * 60u / a.dt;
@@ -398,6 +399,182 @@ __global__ void k_immunization(ImmArgs a) {
     for (uint32_t v = 0; v < nVar; ++v) A.susc[(size_t)v * A.n + i] = fmin_ref(1.0f, susc[v]);
 }
 
+// ------------------------------------------------------------ vaccination rollout
+// Membership of the ten categories of initCategories (immunization.h:229-339),
+// indexed as the reference's pass loop indexes them; location-type ids 2, 3,
+// 4, 12, 14 and 22 are literals there.
+__device__ __forceinline__ bool in_vacc_category(const Agents& A, const Locs& L, uint32_t i, const AgentRec& r, uint32_t g,
+                                                 uint8_t precond) {
+    const uint32_t age = r.age;
+    auto anyEntry = [&](uint32_t kind) {
+        for (uint32_t e = r.csrBegin; e < r.csrBegin + r.csrCount; ++e) {
+            const uint32_t t = A.possType[e];
+            const LocRec lr = L.rec[A.possLoc[e]];
+            bool hit = false;
+            if (kind == 0) hit = t == 4 && (lr.type == 12 || lr.type == 14);
+            else if (kind == 1) hit = (t == 4 || t == 2) && lr.type == 22;
+            else if (kind == 4) hit = t == 4 && lr.essential == 1;
+            else hit = t == 4 && lr.type == 3;
+            if (hit) return true;
+        }
+        return false;
+    };
+    switch (g) {
+        case 0: return anyEntry(0);
+        case 1: return anyEntry(1);
+        case 2: return age >= 60;
+        case 3: return precond > 0 && age >= 18 && age < 60;
+        case 4: return anyEntry(4);
+        case 5: return age > 17 && age < 60;
+        case 6: return precond > 0 && age >= 60;
+        case 7: return anyEntry(7);
+        case 8: return age >= 12 && age < 18;
+        case 9: return age >= 5 && age < 12;
+    }
+    return false;
+}
+
+struct VaccCategoryArgs {
+    Agents A;
+    Locs L;
+    uint64_t seed;
+    uint32_t replica;
+    uint32_t nPasses;
+    uint8_t passes[10];   // group indices in visiting order
+    uint8_t round[10];    // round of each group (its immunizationOrder value)
+    float level[10];      // willingness of each group
+};
+
+// initCategories: an agent is decided by the first visited group it belongs
+// to, with one draw.
+__global__ void k_vacc_categories(VaccCategoryArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.A.n) return;
+    const AgentRec r = a.A.rec[i];
+    const uint8_t pre = a.A.precond[i];
+    uint8_t round = 0;
+    for (uint32_t p = 0; p < a.nPasses; ++p) {
+        const uint32_t g = a.passes[p];
+        if (!in_vacc_category(a.A, a.L, i, r, g, pre)) continue;
+        Stream rs(a.seed, i, 0, PH_IMM_CATEGORY, a.replica);
+        round = rs.unit() < (double)a.level[g] ? a.round[g] : (uint8_t)255;
+        break;
+    }
+    a.A.immRound[i] = round;
+}
+
+__device__ __forceinline__ int age_group_of(uint32_t age) { return age < 90 ? (int)(age / 10) : (age < 200 ? 9 : -1); }
+
+// ages per 10-year group (initAgeGroups, immunization.h:421-479)
+__global__ void k_age_groups(Agents A, unsigned long long* out) {
+    __shared__ unsigned int sh[10];
+    if (threadIdx.x < 10) sh[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        int g = age_group_of(A.rec[i].age);
+        if (g >= 0) atomicAdd(&sh[g], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 10 && sh[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
+}
+
+struct VaccDayArgs {
+    Agents A;
+    Tables T;
+    uint64_t seed;
+    uint32_t replica;
+    uint32_t ts;     // timestamp, 1 at timestamp 0 (immunization.h:485)
+    uint32_t tsKey;  // timestamp of the draws
+    uint32_t spd;    // steps per day
+    uint32_t round;  // booster round / category round
+    uint32_t draw;   // draw index of this pass
+    uint32_t anyRound;
+    float prob[10];
+};
+
+// booster eligibility (immunization.h:707-717)
+__device__ __forceinline__ bool booster_eligible(const VaccDayArgs& a, uint32_t i, const AgentRec& r) {
+    const uint32_t it = a.A.immTs[i];
+    return it > 0 && a.A.immCount[i] >= 1 && (a.ts - it) / a.spd > 4u * 30u && a.T.stWB[r.state] == 0 &&
+           (a.ts >= a.spd * 3u * 30u && r.diagTs < a.ts - a.spd * 4u * 30u);
+}
+
+// first-dose eligibility (immunization.h:849-854)
+__device__ __forceinline__ bool vacc_eligible(const VaccDayArgs& a, uint32_t i, const AgentRec& r) {
+    return a.A.immCount[i] == 0 && a.T.stWB[r.state] == 0 &&
+           ((a.ts < a.spd * 3u * 30u && r.diagTs == 0) || (a.ts >= a.spd * 3u * 30u && r.diagTs < a.ts - a.spd * 3u * 30u));
+}
+
+__global__ void k_booster_count(VaccDayArgs a, unsigned long long* out) {
+    __shared__ unsigned int sh[10];
+    if (threadIdx.x < 10) sh[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.A.n; i += gridDim.x * blockDim.x) {
+        const AgentRec r = a.A.rec[i];
+        int g = age_group_of(r.age);
+        if (g >= 0 && booster_eligible(a, i, r)) atomicAdd(&sh[g], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 10 && sh[threadIdx.x]) atomicAdd(&out[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
+}
+
+// the age groups are disjoint, so one pass applies all ten in the
+// reference's group-by-group order
+__global__ void k_booster_apply(VaccDayArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.A.n) return;
+    const AgentRec r = a.A.rec[i];
+    int g = age_group_of(r.age);
+    if (g < 0 || !booster_eligible(a, i, r)) return;
+    const float prob = a.prob[g];
+    Stream rs(a.seed, i, a.tsKey, PH_VACCINATION, a.replica);
+    rs.draw = a.draw;
+    if (prob == 1.0f || rs.unit() < (double)prob) {
+        a.A.immTs[i] = a.ts;
+        a.A.immCount[i] = (uint8_t)(a.A.immCount[i] + 1);
+    }
+}
+
+// per-thread counts, one LDS atomic per thread and one global atomic per block
+__device__ __forceinline__ void block_count_add(unsigned int mine, unsigned long long* out) {
+    __shared__ unsigned int sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    if (mine) atomicAdd(&sh, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && sh) atomicAdd(out, (unsigned long long)sh);
+}
+
+__global__ void k_vacc_count(VaccDayArgs a, unsigned long long* out) {
+    unsigned int mine = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.A.n; i += gridDim.x * blockDim.x) {
+        const AgentRec r = a.A.rec[i];
+        mine += ((a.A.immRound[i] == a.round || a.anyRound) && vacc_eligible(a, i, r)) ? 1u : 0u;
+    }
+    block_count_add(mine, out);
+}
+
+__global__ void k_vacc_apply(VaccDayArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.A.n) return;
+    if (a.A.immRound[i] != a.round) return;
+    const AgentRec r = a.A.rec[i];
+    if (!vacc_eligible(a, i, r)) return;
+    Stream rs(a.seed, i, a.tsKey, PH_VACCINATION, a.replica);
+    rs.draw = a.draw;
+    if (a.prob[0] == 1.0f || rs.unit() < (double)a.prob[0]) {
+        a.A.immTs[i] = a.ts;
+        a.A.immCount[i] = 1;
+    }
+}
+
+__global__ void k_count_immunized(Agents A, uint32_t ts, unsigned long long* out) {
+    unsigned int mine = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x)
+        mine += A.immTs[i] == ts ? 1u : 0u;
+    block_count_add(mine, out);
+}
+
 // ------------------------------------------------------------ non-COVID hospitalisation
 __constant__ double c_randomHospProbs[84] = {
     0.00017785,  0.000118567, 0.000614453, 0.003174864, 0.001016573, 0.0011019,   0.000926381, 9.52062E-05,
@@ -417,19 +594,19 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
                                 float probabilityMul) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    uint8_t st = A.state[i];
+    uint8_t st = A.rec[i].state;
     if (T.stWB[st] == 4) return;
     Stream rs(seed, i, ts, PH_OTHER_DISEASE, replica);
-    uint8_t age = A.age[i];
+    uint8_t age = A.rec[i].age;
     uint32_t ag = age < 5 ? 0 : age < 15 ? 1 : age < 30 ? 2 : age < 60 ? 3 : age < 70 ? 4 : age < 80 ? 5 : 6;
     uint32_t notSex = A.sex[i] ? 0u : 1u;
     uint8_t pc = A.precond[i];
     uint32_t type = pc == 2 ? 1 : (pc == 4 ? 2 : 0);
-    if (ts > 0 && A.hospUntil[i] == ts) {
+    if (ts > 0 && A.rec[i].hospUntil == ts) {
         if (rs.real(1.0) < c_randomHospProbs[type * 4 * 7 + 2 * 7 + notSex * 7 + ag]) {
             // die(false) (dynamicPPState.cpp:313-325)
             uint8_t d = (uint8_t)T.nonCOVIDDead;
-            A.state[i] = d;
+            A.rec[i].state = d;
             A.days[i] = -1;
             update_meta(A, T, i, d);
             A.worstState[i] = (int8_t)d;
@@ -438,7 +615,7 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
         }
     }
     uint8_t w = T.stWB[st];
-    if (w == 4 || w == 3 || ts < A.hospUntil[i]) return;
+    if (w == 4 || w == 3 || ts < A.rec[i].hospUntil) return;
     (void)rs.real(1.0);  // sudden death: disabled in the reference, draw still made
     const int multDay[4] = {99, 99 + 365, 99 + 2 * 365, 99 + 3 * 365};
     const double mult[4] = {0.82, 0.67, 0.9, 0.9};
@@ -453,8 +630,8 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
         double p = 1.0 / (double)avg;
         unsigned length = (unsigned)geometric(p, rs);
         if (length == 0) length = 1;
-        A.hospTs[i] = ts;
-        A.hospUntil[i] = ts + length * 24u * 60u / dt;
+        A.rec[i].hospTs = ts;
+        A.rec[i].hospUntil = ts + length * 24u * 60u / dt;
     }
 }
 
@@ -463,11 +640,11 @@ __global__ void k_count_covid_hospital(Agents A, Locs L, Tables T, uint32_t ts, 
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool c = false;
     if (i < A.n) {
-        uint8_t st = A.state[i];
+        uint8_t st = A.rec[i].state;
         if (dueWith == 0) c = st >= 6 && st <= 8;
         else
-            c = T.stInfected[st] && L.type[A.location[i]] == T.lt.hospital &&
-                !(T.stWB[st] < 3 && ts > 0 && A.hospUntil[i] == ts);
+            c = T.stInfected[st] && L.rec[A.location[i]].type == T.lt.hospital &&
+                !(T.stWB[st] < 3 && ts > 0 && A.rec[i].hospUntil == ts);
     }
     unsigned long long m = __ballot(c);
     if ((threadIdx.x & 63u) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
@@ -493,14 +670,14 @@ __global__ void __launch_bounds__(256) k_stats(Agents A, Locs L, Tables T, uint3
     uint32_t stride = gridDim.x * blockDim.x;
     unsigned int c[14] = {0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
-        uint8_t st = A.state[i];
+        uint8_t st = A.rec[i].state;
         uint8_t w = T.stWB[st];
-        uint8_t fl = A.flags[i];
+        uint8_t fl = A.rec[i].flags;
         bool inf = T.stInfected[st] != 0;
         atomicAdd(&sh[S_STATE + st], 1ull);
-        c[S_HOSP] += (w != 4 && w != 3 && ts < A.hospUntil[i]);
+        c[S_HOSP] += (w != 4 && w != 3 && ts < A.rec[i].hospUntil);
         c[S_STAYED] += (fl & F_STAYED_HOME) ? 1 : 0;
-        bool isQ = A.quarTs[i] <= ts && A.quarUntil[i] > ts;
+        bool isQ = A.quarTs[i] <= ts && A.rec[i].quarUntil > ts;
         c[S_Q] += isQ;
         c[S_QT] += isQ && inf;
         c[S_NQ] += !isQ && inf;
@@ -516,9 +693,9 @@ __global__ void __launch_bounds__(256) k_stats(Agents A, Locs L, Tables T, uint3
             c[S_HCI] += inf || (fl & F_QUARANTINED);
             c[S_HCE] += newInf;
         }
-        c[S_INFH] += inf && L.type[A.location[i]] == T.lt.hospital;
+        c[S_INFH] += inf && L.rec[A.location[i]].type == T.lt.hospital;
         c[S_VNI] += ic == 0 && mc > 0;
-        if (inf) atomicAdd(&sh[S_VARINF + A.variant[i]], 1ull);
+        if (inf) atomicAdd(&sh[S_VARINF + A.rec[i].variant], 1ull);
         uint8_t sv = A.statVar[i];
         for (uint32_t v = 0; v < nVar; ++v) {
             if ((sv >> v) & 1u) atomicAdd(&sh[S_INFV + v], 1ull);
@@ -549,30 +726,30 @@ __global__ void k_plan_reset(PlanArgs a) {
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = gid; i < a.A.n; i += stride) {
         // stepsUntilMove := curfewEnd at this midnight step
-        a.A.wake[i] = a.ts + a.curfewEnd;
-        a.A.flags[i] = (uint8_t)((a.A.flags[i] | F_STAYED_HOME) & ~F_NO_WORK);
+        a.A.wake[i] = a.curfewEnd < 1440u / a.dt ? (uint16_t)a.curfewEnd : WAKE_NONE;
+        a.A.rec[i].flags = (uint8_t)((a.A.rec[i].flags | F_STAYED_HOME) & ~F_NO_WORK);
     }
-    for (uint32_t l = gid; l < a.L.n; l += stride) a.L.noWork[l] = 0u;
+    for (uint32_t l = gid; l < a.L.n; l += stride) a.L.rec[l].noWork = 0u;
 }
 
 // checkSchoolQuarantine (movementPolicies.h:1285-1310), one thread per school
 __global__ void k_plan_school(PlanArgs a) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.L.nSchools) return;
-    uint32_t* q = a.L.quarUntil;
+    LocRec* q = a.L.rec;
     uint32_t school = a.L.schools[s];
-    if (q[school] > a.ts) return;
+    if (q[school].quarUntil > a.ts) return;
     uint32_t counter = 0, mx = 0;
     for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) {
-        uint32_t cl = a.L.classrooms[c];
-        if (q[cl] > a.ts) {
+        uint32_t cu = q[a.L.classrooms[c]].quarUntil;
+        if (cu > a.ts) {
             counter++;
-            mx = mx > q[cl] ? mx : q[cl];
+            mx = mx > cu ? mx : cu;
         }
     }
     if (counter > 1) {
-        q[school] = mx;
-        for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) q[a.L.classrooms[c]] = mx;
+        q[school].quarUntil = mx;
+        for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) q[a.L.classrooms[c]].quarUntil = mx;
     }
 }
 
@@ -581,23 +758,23 @@ __global__ void k_plan_schoolwork(PlanArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    uint8_t fl = A.flags[i];
+    uint8_t fl = A.rec[i].flags;
     if (fl & F_QUARANTINED) return;
-    if (A.diagTs[i] > 0 && !a.T.stInfected[A.state[i]]) return;
+    if (A.rec[i].diagTs > 0 && !a.T.stInfected[A.rec[i].state]) return;
     const pansim_location_types& lt = a.T.lt;
     Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
     uint32_t wl = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
     uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
     uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
-    const uint32_t* q = a.L.quarUntil;
+    const LocRec* q = a.L.rec;
     uint32_t until = 0;
-    if (sl != NOLOC && q[sl] > a.ts) until = q[sl];
-    else if (cl != NOLOC && q[cl] > a.ts) until = q[cl];
-    else if (wl != NOLOC && q[wl] > a.ts) until = q[wl];
+    if (sl != NOLOC && q[sl].quarUntil > a.ts) until = q[sl].quarUntil;
+    else if (cl != NOLOC && q[cl].quarUntil > a.ts) until = q[cl].quarUntil;
+    else if (wl != NOLOC && q[wl].quarUntil > a.ts) until = q[wl].quarUntil;
     if (until > 0) {
-        A.flags[i] = (uint8_t)(fl | F_QUARANTINED);
+        A.rec[i].flags = (uint8_t)(fl | F_QUARANTINED);
         A.quarTs[i] = a.ts;
-        A.quarUntil[i] = until;
+        A.rec[i].quarUntil = until;
         A.daysInQ[i] += (until - a.ts) / (24u * 60u / a.dt);
     }
 }
@@ -607,13 +784,13 @@ __global__ void k_plan_close(PlanArgs a) {
     uint32_t stride = gridDim.x * blockDim.x;
     const pansim_location_types& lt = a.T.lt;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < a.L.n; l += stride) {
-        uint32_t q = a.L.quarUntil[l];
-        uint32_t t = a.L.type[l];
+        uint32_t q = a.L.rec[l].quarUntil;
+        uint32_t t = a.L.rec[l].type;
         if (q > a.ts && (t == lt.school || t == lt.classroom)) {
-            a.L.closedUntil[l] = q;
-            a.L.quarUntil[l] = 0;
+            a.L.rec[l].closedUntil = q;
+            a.L.rec[l].quarUntil = 0;
         } else if (q > a.ts && t == lt.work) {
-            a.L.quarUntil[l] = 0;
+            a.L.rec[l].quarUntil = 0;
         }
     }
 }
@@ -625,21 +802,21 @@ __global__ void k_plan_underage(PlanArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    if (A.age[i] > 14) return;
+    if (A.rec[i].age > 14) return;
     const pansim_location_types& lt = a.T.lt;
     bool flag = false;
-    if (A.flags[i] & F_QUARANTINED) flag = true;
+    if (A.rec[i].flags & F_QUARANTINED) flag = true;
     else {
         Stream rs(a.seed, i, a.ts, PH_PLAN_UNDERAGE, a.replica);
         uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
         uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
-        bool tooOld = A.age[i] >= a.schoolAge;
-        flag = tooOld || (sl != NOLOC && (a.L.open[sl] == 0 || a.L.closedUntil[sl] > a.ts)) ||
-               (cl != NOLOC && (a.L.open[cl] == 0 || a.L.closedUntil[cl] > a.ts));
+        bool tooOld = A.rec[i].age >= a.schoolAge;
+        flag = tooOld || (sl != NOLOC && (a.L.rec[sl].open == 0 || a.L.rec[sl].closedUntil > a.ts)) ||
+               (cl != NOLOC && (a.L.rec[cl].open == 0 || a.L.rec[cl].closedUntil > a.ts));
     }
     if (flag) {
         uint32_t h = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
-        if (h != NOLOC) a.L.noWork[h] = NOWORK_FLAGGED;
+        if (h != NOLOC) a.L.rec[h].noWork = NOWORK_FLAGGED;
     }
 }
 
@@ -649,35 +826,35 @@ __global__ void k_plan_nowork_claim(PlanArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    if (!(A.age[i] > 26 && A.age[i] < 65)) return;
+    if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
     uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
     // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent + 1)
-    if (h != NOLOC && a.L.noWork[h] != 0u) atomicMin(&a.L.noWork[h], i + 1u);
+    if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, i + 1u);
 }
 
 __global__ void k_plan_nowork_resolve(PlanArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    if (!(A.age[i] > 26 && A.age[i] < 65)) return;
+    if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
     uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
-    if (h != NOLOC && a.L.noWork[h] == i + 1u) A.flags[i] = (uint8_t)(A.flags[i] | F_NO_WORK);
+    if (h != NOLOC && a.L.rec[h].noWork == i + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
 }
 
 // ------------------------------------------------------------ closures
 __global__ void k_set_open(Locs L, const uint32_t* types, uint32_t n, uint8_t open) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
-        if (L.essential[l] == 1) continue;
+        if (L.rec[l].essential == 1) continue;
         for (uint32_t k = 0; k < n; ++k)
-            if (L.type[l] == types[k]) L.open[l] = open;
+            if (L.rec[l].type == types[k]) L.rec[l].open = open;
     }
 }
 
 __global__ void k_scale_inf(Locs L, uint32_t homeType, double factor, uint32_t divide) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride)
-        if (L.type[l] != homeType) L.inf[l] = divide ? (float)((double)L.inf[l] / factor) : (float)((double)L.inf[l] * factor);
+        if (L.rec[l].type != homeType) L.dyn[l].inf = divide ? (float)((double)L.dyn[l].inf / factor) : (float)((double)L.dyn[l].inf * factor);
 }
 
 // ExposeToMutation (closurePolicies.h:356-390)
@@ -690,8 +867,8 @@ __global__ void k_expose_mutation(Agents A, Tables T, uint64_t seed, uint32_t re
     Stream rs(seed, i, ts, PH_MUTATION, replica);
     if (rs.unit() < fraction * (double)s) {
         uint8_t fi = (uint8_t)T.firstInfected;
-        A.state[i] = fi;
-        A.variant[i] = (uint8_t)v;
+        A.rec[i].state = fi;
+        A.rec[i].variant = (uint8_t)v;
         A.days[i] = -2;
         update_meta(A, T, i, fi);
         A.infTs[i] = ts;

@@ -3,6 +3,7 @@
 // of every per-step and per-midnight routine.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -41,6 +42,28 @@ struct pansim_ctx {
     pansim_event* evDev = nullptr;
     uint32_t evOffCap = 0, evCap = 0;
     uint32_t nEvents = 0;
+    int2* evWinDev = nullptr;
+    uint32_t dt = 10;
+    uint32_t wakeDay = 0;  // first step of the day the stored wake steps count from
+};
+
+// A batch of replicas stepped together (include/pansim_hip.h): the member
+// contexts keep their arrays; the batch owns the per-replica argument tables
+// and the stream every member works on while it is batched.
+struct pansim_batch {
+    std::vector<pansim_ctx*> reps;
+    std::vector<hipStream_t> ownStreams;  // members' streams, restored on destroy
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint32_t dt = 10;
+    uint32_t pendPar = 0;
+    uint32_t moveBlocks = 0, infBlocks = 0;
+    ps::dev::MoveArgs* dMove = nullptr;
+    ps::dev::InfectArgs* dInf = nullptr;
+    uint32_t* dTables = nullptr;  // moveRep | moveFirst | infRep | infFirst
+    ps::dev::MoveArgs* hMove = nullptr;  // pinned staging of the per-day arguments
+    ps::dev::InfectArgs* hInf = nullptr;
+    bool haveDay = false;
 };
 
 namespace {
@@ -88,6 +111,23 @@ int check_launch(pansim_ctx* c, const char* what) {
 }
 
 unsigned persistent_blocks(pansim_ctx* c) { return (unsigned)c->numCUs * 8u; }
+
+// Schedule offsets must be non-decreasing and stay within the events.
+bool offsets_ok(const uint32_t* off, uint32_t nOff, uint32_t nEvents) {
+    for (uint32_t k = 0; k + 1 < nOff; ++k)
+        if (off[k] > off[k + 1]) return false;
+    return nOff > 0 && off[nOff - 1] <= nEvents;
+}
+
+// Recomputes the per-step schedule windows after the schedules changed.
+int compute_windows(pansim_ctx* c) {
+    const uint32_t spd = 1440u / c->dt;
+    const uint32_t total = c->T.nOff * spd;
+    if (total == 0) return 0;
+    hipLaunchKernelGGL(k_event_windows, dim3(blocks_for(total)), dim3(256), 0, c->stream, c->T.evOff, c->T.ev, c->T.nOff, spd,
+                       c->dt, c->evWinDev);
+    return check_launch(c, "event windows");
+}
 
 }  // namespace
 
@@ -171,6 +211,10 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(upload(c, (float**)&T.lenMax, t->length_max, (size_t)NP * NS * 7));
     TRY(upload(c, (double**)&T.lenP, t->length_p, (size_t)NP * NS * 7));
     c->evOffCap = d->n_agent_types * 28 + 1;
+    if (!offsets_ok(t->event_offset, c->evOffCap, d->n_events)) {
+        c->err = "pansim_create: schedule offsets out of range";
+        return bail(PANSIM_E_ARG);
+    }
     c->evCap = d->n_events + 4096;
     TRY(dalloc(c, &c->evOffDev, c->evOffCap));
     TRY(dalloc(c, &c->evDev, c->evCap));
@@ -180,73 +224,128 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     T.evOff = c->evOffDev;
     T.ev = c->evDev;
     c->nEvents = d->n_events;
+    c->dt = time_step;
+    T.nOff = c->evOffCap - 1;
+    TRY(dalloc(c, &c->evWinDev, (size_t)T.nOff * (1440u / time_step)));
+    T.evWin = c->evWinDev;
+    if ((r = compute_windows(c)) != 0) return bail(r);
+    // ---- input checks (indexes the kernels rely on)
+    for (uint32_t k = 0; k < NL; ++k)
+        if (l->type[k] >= LOC_TYPE_NONE) {
+            c->err = "pansim_create: location type ids must be below 255";
+            return bail(PANSIM_E_UNSUPPORTED);
+        }
+    if (l->cemetery >= NL) {
+        c->err = "pansim_create: cemetery index out of range";
+        return bail(PANSIM_E_ARG);
+    }
+    if (a->loc_offset[0] != 0 || a->loc_offset[N] != d->n_csr) {
+        c->err = "pansim_create: inconsistent possible-location offsets";
+        return bail(PANSIM_E_ARG);
+    }
+    for (uint32_t k = 0; k < N; ++k)
+        if (a->location[k] >= NL || a->loc_offset[k + 1] < a->loc_offset[k] || a->loc_offset[k + 1] - a->loc_offset[k] > 0xFFFFu) {
+            c->err = "pansim_create: agent location or location list out of range";
+            return bail(PANSIM_E_ARG);
+        }
+    for (uint32_t k = 0; k < d->n_csr; ++k)
+        if (a->possible_locations[k] >= NL) {
+            c->err = "pansim_create: possible location out of range";
+            return bail(PANSIM_E_ARG);
+        }
     // ---- locations
     Locs& L = c->L;
     L.n = NL;
     L.cemetery = l->cemetery;
     L.nSchools = d->n_schools;
     {
-        std::vector<uint8_t> ty(NL);
-        for (uint32_t i = 0; i < NL; ++i) ty[i] = (uint8_t)l->type[i];
-        TRY(upload(c, &L.type, ty.data(), NL));
+        std::vector<LocRec> lr(NL);
+        std::vector<LocDyn> ld(NL);
+        for (uint32_t k = 0; k < NL; ++k) {
+            lr[k].quarUntil = 0;
+            lr[k].closedUntil = 0;
+            lr[k].noWork = 0;
+            lr[k].type = (uint8_t)l->type[k];
+            lr[k].open = l->open[k];
+            lr[k].essential = l->essential[k];
+            lr[k].flag = 0;
+            ld[k].sum0 = 0;
+            ld[k].count = 0;
+            ld[k].inf = l->infectiousness[k];
+        }
+        TRY(upload(c, &L.rec, lr.data(), NL));
+        TRY(upload(c, &L.dyn, ld.data(), NL));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
-    TRY(upload(c, &L.open, l->open, NL));
-    TRY(upload(c, &L.essential, l->essential, NL));
-    TRY(dalloc(c, &L.flag, NL));
-    TRY(dalloc(c, &L.closedUntil, NL));
-    TRY(dalloc(c, &L.quarUntil, NL));
-    TRY(dalloc(c, &L.noWork, NL));
-    TRY(dalloc(c, &L.count, NL));
-    TRY(upload(c, &L.inf, l->infectiousness, NL));
-    TRY(dalloc(c, &L.sum, (size_t)V * NL));
+    TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));
     TRY(dalloc(c, &L.pendKey, NL));
     TRY(upload(c, (uint32_t**)&L.schools, l->schools, d->n_schools));
     TRY(upload(c, (uint32_t**)&L.classrooms, l->classrooms, d->n_classrooms));
     TRY(upload(c, (uint32_t**)&L.classOff, l->classroom_offsets, d->n_schools + 1));
-    HIP_OR_FAIL(c, hipMemsetAsync(L.flag, 0, NL, c->stream));
-    HIP_OR_FAIL(c, hipMemsetAsync(L.closedUntil, 0, NL * sizeof(uint32_t), c->stream));
-    HIP_OR_FAIL(c, hipMemsetAsync(L.quarUntil, 0, NL * sizeof(uint32_t), c->stream));
-    HIP_OR_FAIL(c, hipMemsetAsync(L.noWork, 0, NL * sizeof(uint32_t), c->stream));
     HIP_OR_FAIL(c, hipMemsetAsync(L.pendKey, 0, NL * sizeof(unsigned long long), c->stream));
     // ---- agents
     Agents& A = c->A;
     A.n = N;
-    TRY(upload(c, &A.state, a->state, N));
-    TRY(dalloc(c, &A.variant, N));
-    TRY(upload(c, &A.prog, a->progression, N));
-    TRY(upload(c, &A.type, a->agent_type, N));
-    TRY(upload(c, &A.age, a->age, N));
-    TRY(upload(c, &A.sex, a->sex, N));
-    TRY(upload(c, &A.precond, a->precond, N));
     {
-        std::vector<uint8_t> fl(N);
+        std::vector<AgentRec> ar(N);
         uint32_t hcw = 0;
-        for (uint32_t i = 0; i < N; ++i) {
-            fl[i] = (uint8_t)((a->diagnosed[i] ? F_DIAGNOSED : 0) | (a->healthcare_worker[i] ? F_HCW : 0));
-            hcw += a->healthcare_worker[i] ? 1 : 0;
+        const uint32_t homeT = t->lt.home;
+        for (uint32_t k = 0; k < N; ++k) {
+            AgentRec& r = ar[k];
+            r.quarUntil = r.diagTs = r.hospTs = r.hospUntil = 0;  // set by k_init_agents
+            r.csrBegin = a->loc_offset[k];
+            r.csrCount = (uint16_t)(a->loc_offset[k + 1] - a->loc_offset[k]);
+            r.home = NOLOC;
+            for (uint32_t e = a->loc_offset[k]; e < a->loc_offset[k + 1]; ++e)
+                if (a->possible_types[e] == homeT) {
+                    r.home = a->possible_locations[e];
+                    break;
+                }
+            r.type = a->agent_type[k];
+            r.state = a->state[k];
+            r.flags = (uint8_t)((a->diagnosed[k] ? F_DIAGNOSED : 0) | (a->healthcare_worker[k] ? F_HCW : 0));
+            r.variant = 0;
+            r.age = a->age[k];
+            r.pad = 0;
+            hcw += a->healthcare_worker[k] ? 1 : 0;
         }
         c->hcwCount = hcw;
-        TRY(upload(c, &A.flags, fl.data(), N));
+        TRY(upload(c, &A.rec, ar.data(), N));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
+    TRY(upload(c, &A.prog, a->progression, N));
+    TRY(upload(c, &A.sex, a->sex, N));
+    TRY(upload(c, &A.precond, a->precond, N));
     TRY(dalloc(c, &A.days, N));
     TRY(upload(c, &A.location, a->location, N));
-    TRY(dalloc(c, &A.wake, N));
+    {
+        // padded to whole movement tiles with WAKE_NONE: the scan reads 16-byte vectors
+        const size_t nw = ((size_t)N + MOVE_TILE - 1) / MOVE_TILE * MOVE_TILE;
+        TRY(dalloc(c, &A.wake, nw));
+        HIP_OR_FAIL(c, hipMemsetAsync(A.wake, 0xFF, nw * sizeof(uint16_t), c->stream));
+    }
     TRY(dalloc(c, &A.lastTest, N));
     TRY(upload(c, &A.scal, a->scaling_age_sex, N));
     TRY(dalloc(c, &A.susc, (size_t)V * N));
     TRY(dalloc(c, &A.slot6, (size_t)V * N));
-    uint32_t** u32s[] = {&A.infTs, &A.infLoc, &A.worstTs, &A.worstEndTs, &A.diagTs, &A.quarTs,
-                         &A.quarUntil, &A.daysInQ, &A.hospTs, &A.hospUntil, &A.immTs, &A.recTs};
+    uint32_t** u32s[] = {&A.infTs, &A.infLoc, &A.worstTs, &A.worstEndTs, &A.quarTs, &A.daysInQ, &A.immTs, &A.recTs};
     for (auto p : u32s) TRY(dalloc(c, p, N));
     TRY(dalloc(c, &A.immCount, N));
+    TRY(dalloc(c, &A.immRound, N));
+    HIP_OR_FAIL(c, hipMemsetAsync(A.immRound, 0, N, c->stream));
     TRY(dalloc(c, &A.statVar, N));
     TRY(dalloc(c, &A.infCount, N));
     TRY(dalloc(c, &A.worstState, N));
-    TRY(upload(c, (uint32_t**)&A.locOffset, a->loc_offset, (size_t)N + 1));
     TRY(upload(c, (uint32_t**)&A.possLoc, a->possible_locations, d->n_csr));
-    TRY(upload(c, (uint8_t**)&A.possType, a->possible_types, d->n_csr));
+    {
+        // padded by 16 zero bytes: the movement kernel reads the types with aligned 4-byte loads
+        uint8_t* pt = nullptr;
+        TRY(dalloc(c, &pt, (size_t)d->n_csr + 16));
+        HIP_OR_FAIL(c, hipMemsetAsync(pt, 0, (size_t)d->n_csr + 16, c->stream));
+        if (d->n_csr)
+            HIP_OR_FAIL(c, hipMemcpyAsync(pt, a->possible_types, d->n_csr, hipMemcpyHostToDevice, c->stream));
+        A.possType = pt;
+    }
     // ---- scratch
     c->S.pendCap = d->n_locations + 16u;
     TRY(dalloc(c, &c->S.pendList, c->S.pendCap));
@@ -358,8 +457,7 @@ int pansim_movement_counters(pansim_ctx* c, uint64_t* evaluated, uint64_t* reloc
     return 0;
 }
 
-int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
-    if (!c || !p) return PANSIM_E_ARG;
+static MoveArgs move_args(pansim_ctx* c, const pansim_step_params* p) {
     MoveArgs m;
     m.A = c->A;
     m.L = c->L;
@@ -368,11 +466,36 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     m.p = *p;
     m.seed = c->seed;
     m.replica = c->replica;
-    const uint32_t minutes = (p->timestamp % (1440u / p->time_step)) * p->time_step;
-    m.curfewStep = (p->enable_curfew && p->curfew_begin == minutes / p->time_step) ? 1u : 0u;
+    m.stepOfDay = p->timestamp % (1440u / c->dt);
+    m.curfewStep = (p->enable_curfew && p->curfew_begin == m.stepOfDay) ? 1u : 0u;
     m.nEvents = c->nEvents;
     m.counters = c->moveCounters;
     m.pendPar = c->pendPar;
+    return m;
+}
+
+static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, float k, float season, uint32_t dt) {
+    InfectArgs a;
+    a.A = c->A;
+    a.L = c->L;
+    a.T = c->T;
+    a.S = c->S;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.timestamp = timestamp;
+    a.mask = mask & ((1u << c->T.nVar) - 1u);
+    a.k = k;
+    a.season = season;
+    a.dt = (float)dt;
+    return a;
+}
+
+int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
+    if (!c || !p) return PANSIM_E_ARG;
+    if (p->time_step != c->dt) return fail(c, PANSIM_E_ARG, "movement_step: time step differs from the one given at creation");
+    MoveArgs m = move_args(c, p);
+    if (p->timestamp - m.stepOfDay != c->wakeDay)
+        return fail(c, PANSIM_E_ARG, "movement_step: pansim_plan_locations must run at the start of every day");
     c->pendPar ^= 1u;
     if (c->timing && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
@@ -386,18 +509,7 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
 
 int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, float k, float season, uint32_t dt) {
     if (!c) return PANSIM_E_ARG;
-    InfectArgs a;
-    a.A = c->A;
-    a.L = c->L;
-    a.T = c->T;
-    a.S = c->S;
-    a.seed = c->seed;
-    a.replica = c->replica;
-    a.timestamp = timestamp;
-    a.mask = mask & ((1u << c->T.nVar) - 1u);
-    a.k = k;
-    a.season = season;
-    a.dt = (float)dt;
+    InfectArgs a = infect_args(c, timestamp, mask, k, season, dt);
     unsigned grid = blocks_for(c->A.n, INFECT_TILE);
     if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
     hipLaunchKernelGGL(k_infect, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
@@ -432,6 +544,8 @@ int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
     a.curfewEnd = p->curfew_end;
     a.schoolAge = p->school_age_restriction;
     const unsigned gN = blocks_for(c->A.n);
+    if (p->time_step != c->dt) return fail(c, PANSIM_E_ARG, "plan_locations: time step differs from the one given at creation");
+    c->wakeDay = p->timestamp - p->timestamp % (1440u / c->dt);
     hipLaunchKernelGGL(k_plan_reset, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a);
     if (p->quarantine_policy > 3 && c->L.nSchools)
         hipLaunchKernelGGL(k_plan_school, dim3(blocks_for(c->L.nSchools)), dim3(256), 0, c->stream, a);
@@ -510,6 +624,127 @@ int pansim_immunization_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const float
     }
     hipLaunchKernelGGL(k_immunization, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
     return check_launch(c, "immunization");
+}
+
+int pansim_vaccination_init(pansim_ctx* c, const int32_t order[10], const float group_level[10]) {
+    if (!c || !order || !group_level) return PANSIM_E_ARG;
+    // the willingness of group g is vaccinationGroupLevel[levelOf[g]], as the
+    // reference's category lambdas capture it (immunization.h:238-322)
+    static const int levelOf[10] = {0, 1, 3, 4, 5, 7, 2, 6, 8, 9};
+    VaccCategoryArgs a;
+    a.A = c->A;
+    a.L = c->L;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.nPasses = 0;
+    for (int o = 1; o <= 10; ++o)
+        for (int g = 0; g < 10; ++g)
+            if (order[g] == o) a.passes[a.nPasses++] = (uint8_t)g;
+    for (int g = 0; g < 10; ++g) {
+        if (order[g] > 10) return fail(c, PANSIM_E_ARG, "immunizationOrder values have to be at most 10");
+        a.round[g] = (uint8_t)(order[g] > 0 ? order[g] : 0);
+        a.level[g] = group_level[levelOf[g]];
+    }
+    hipLaunchKernelGGL(k_vacc_categories, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+    return check_launch(c, "vaccination_init");
+}
+
+int pansim_age_group_counts(pansim_ctx* c, uint32_t out[10]) {
+    if (!c || !out) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, 10 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_age_groups, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->A, c->counters);
+    int r = check_launch(c, "age_group_counts");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (int g = 0; g < 10; ++g) out[g] = (uint32_t)c->hostCounters[g];
+    return 0;
+}
+
+static VaccDayArgs vacc_day_args(pansim_ctx* c, uint32_t ts, uint32_t dt) {
+    VaccDayArgs a;
+    a.A = c->A;
+    a.T = c->T;
+    a.seed = c->seed;
+    a.replica = c->replica;
+    a.tsKey = ts;
+    a.ts = ts == 0 ? 1u : ts;
+    a.spd = 24u * 60u / dt;
+    a.round = a.draw = a.anyRound = 0;
+    for (int g = 0; g < 10; ++g) a.prob[g] = 0.0f;
+    return a;
+}
+
+int pansim_booster_round(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t round, uint32_t available,
+                         const float age_group_frac[10]) {
+    if (!c || !age_group_frac || dt == 0) return PANSIM_E_ARG;
+    if (available == 0) return 0;
+    VaccDayArgs a = vacc_day_args(c, ts, dt);
+    a.round = round;
+    a.draw = round;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, 10 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_booster_count, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a, c->counters);
+    int r = check_launch(c, "booster_count");
+    if (r) return r;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (int g = 0; g < 10; ++g) {
+        const uint32_t count = (uint32_t)c->hostCounters[g];
+        const uint32_t availG = (uint32_t)((float)available * age_group_frac[g]);
+        a.prob[g] = count < availG ? 1.0f : (float)availG / (float)count;
+    }
+    hipLaunchKernelGGL(k_booster_apply, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+    return check_launch(c, "booster_apply");
+}
+
+int pansim_vaccinate(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t available, uint32_t* current_category,
+                     uint32_t* immunized_today) {
+    if (!c || !current_category || dt == 0) return PANSIM_E_ARG;
+    VaccDayArgs a = vacc_day_args(c, ts, dt);
+    uint32_t& cat = *current_category;
+    auto count_round = [&](uint32_t cc, uint32_t any, uint32_t* out) -> int {
+        a.round = cc;
+        a.anyRound = any;
+        HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_vacc_count, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a, c->counters);
+        int r = check_launch(c, "vaccination_count");
+        if (r) return r;
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                      c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        *out = (uint32_t)c->hostCounters[0];
+        return 0;
+    };
+    while (available > 0 && cat < 10) {
+        uint32_t count = 0;
+        while (count == 0 && cat <= 10) {
+            const uint32_t cc = cat + 1;  // rounds start at 1; round 11 counts every agent
+            int r = count_round(cc, cc == 11 ? 1u : 0u, &count);
+            if (r) return r;
+            if (count == 0) cat++;
+        }
+        a.round = cat + 1;
+        a.anyRound = 0;
+        a.draw = 32u + cat;
+        a.prob[0] = count < available ? 1.0f : (float)available / (float)count;
+        hipLaunchKernelGGL(k_vacc_apply, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, a);
+        int r = check_launch(c, "vaccination_apply");
+        if (r) return r;
+        available = count < available ? available - count : 0;
+    }
+    if (immunized_today) {
+        HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_count_immunized, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->A, a.ts, c->counters);
+        int r = check_launch(c, "count_immunized");
+        if (r) return r;
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                      c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        *immunized_today = (uint32_t)c->hostCounters[0];
+    }
+    return 0;
 }
 
 int pansim_other_disease_day(pansim_ctx* c, uint32_t ts, uint32_t dt, float probabilityMul) {
@@ -621,17 +856,23 @@ int pansim_expose_mutation(pansim_ctx* c, uint32_t ts, uint32_t v, double fracti
 int pansim_upload_schedules(pansim_ctx* c, const uint32_t* off, uint32_t nOff, const pansim_event* ev, uint32_t nEv) {
     if (!c || !off || nOff != c->evOffCap) return PANSIM_E_ARG;
     if (nEv > c->evCap) return fail(c, PANSIM_E_ARG, "schedule grew beyond its reserved capacity");
+    if (!offsets_ok(off, nOff, nEv)) return fail(c, PANSIM_E_ARG, "schedule offsets out of range");
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     HIP_OR_FAIL(c, hipMemcpy(c->evOffDev, off, nOff * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (nEv) HIP_OR_FAIL(c, hipMemcpy(c->evDev, ev, nEv * sizeof(pansim_event), hipMemcpyHostToDevice));
     c->nEvents = nEv;
+    return compute_windows(c);
     return 0;
 }
 
 int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_t n) {
     if (!c || !out || v >= c->T.nVar || n < c->L.n) return PANSIM_E_ARG;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    HIP_OR_FAIL(c, hipMemcpy(out, c->L.sum + (size_t)v * c->L.n, c->L.n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (v == 0)
+        HIP_OR_FAIL(c, hipMemcpy2D(out, sizeof(int64_t), &c->L.dyn[0].sum0, sizeof(LocDyn), sizeof(int64_t), c->L.n,
+                                   hipMemcpyDeviceToHost));
+    else
+        HIP_OR_FAIL(c, hipMemcpy(out, c->L.sumX + (size_t)(v - 1) * c->L.n, c->L.n * sizeof(int64_t), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -652,15 +893,28 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
         for (uint32_t i = 0; i < cnt; ++i) out[i] = isSigned ? (uint32_t)(int32_t)(int8_t)tmp[i] : tmp[i];
         return 0;
     };
+    // one member of an array of records: a strided copy
+    auto getMember = [&](const void* base, size_t pitch, size_t offset, size_t width, uint32_t cnt) -> int {
+        if (n < cnt) return fail(c, PANSIM_E_ARG, "output buffer too small");
+        std::vector<uint8_t> tmp((size_t)cnt * width);
+        HIP_OR_FAIL(c, hipMemcpy2D(tmp.data(), width, (const uint8_t*)base + offset, pitch, width, cnt, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < cnt; ++i) {
+            if (width == 4) memcpy(&out[i], &tmp[(size_t)i * 4], 4);
+            else out[i] = tmp[i];
+        }
+        return 0;
+    };
+#define AGENT_MEMBER(m) getMember(c->A.rec, sizeof(AgentRec), offsetof(AgentRec, m), sizeof(AgentRec::m), N)
+#define LOC_MEMBER(arr, T, m) getMember(c->L.arr, sizeof(T), offsetof(T, m), sizeof(T::m), NL)
     auto getFlag = [&](uint8_t bit) -> int {
-        int r = get8(c->A.flags, N, false);
+        int r = AGENT_MEMBER(flags);
         if (r) return r;
         for (uint32_t i = 0; i < N; ++i) out[i] = (out[i] & bit) ? 1u : 0u;
         return 0;
     };
     const Agents& A = c->A;
-    if (f == "state") return get8(A.state, N, false);
-    if (f == "variant") return get8(A.variant, N, false);
+    if (f == "state") return AGENT_MEMBER(state);
+    if (f == "variant") return AGENT_MEMBER(variant);
     if (f == "days") {
         if (n < N) return PANSIM_E_ARG;
         std::vector<int16_t> tmp(N);
@@ -670,9 +924,10 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
     }
     if (f == "location") return get32(A.location, N);
     if (f == "stepsUntilMove") {
-        int r = get32(A.wake, N);
-        if (r) return r;
-        for (uint32_t i = 0; i < N; ++i) out[i] = out[i] == NEVER ? NEVER : out[i] - ts;
+        if (n < N) return fail(c, PANSIM_E_ARG, "output buffer too small");
+        std::vector<uint16_t> tmp(N);
+        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), A.wake, N * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < N; ++i) out[i] = tmp[i] == WAKE_NONE ? NEVER : c->wakeDay + tmp[i] - ts;
         return 0;
     }
     if (f == "diagnosed") return getFlag(F_DIAGNOSED);
@@ -685,21 +940,26 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
     if (f == "infectedLocation") return get32(A.infLoc, N);
     if (f == "worstStateTimestamp") return get32(A.worstTs, N);
     if (f == "worstStateEndTimestamp") return get32(A.worstEndTs, N);
-    if (f == "diagnosedTimestamp") return get32(A.diagTs, N);
+    if (f == "diagnosedTimestamp") return AGENT_MEMBER(diagTs);
     if (f == "quarantinedTimestamp") return get32(A.quarTs, N);
-    if (f == "quarantinedUntilTimestamp") return get32(A.quarUntil, N);
+    if (f == "quarantinedUntilTimestamp") return AGENT_MEMBER(quarUntil);
     if (f == "daysInQuarantine") return get32(A.daysInQ, N);
-    if (f == "hospitalizedTimestamp") return get32(A.hospTs, N);
-    if (f == "hospitalizedUntilTimestamp") return get32(A.hospUntil, N);
+    if (f == "hospitalizedTimestamp") return AGENT_MEMBER(hospTs);
+    if (f == "hospitalizedUntilTimestamp") return AGENT_MEMBER(hospUntil);
     if (f == "recoveredTimestamp") return get32(A.recTs, N);
+    if (f == "immunizationTimestamp") return get32(A.immTs, N);
+    if (f == "immunizationCount") return get8(A.immCount, N, false);
+    if (f == "vaccinationRound") return get8(A.immRound, N, false);
     if (f == "infectedCount") return get8(A.infCount, N, false);
     if (f == "statsVariant") return get8(A.statVar, N, false);
     if (f == "worstState") return get8(A.worstState, N, true);
-    if (f == "locQuarantineUntil") return get32(c->L.quarUntil, NL);
-    if (f == "locClosedUntil") return get32(c->L.closedUntil, NL);
-    if (f == "locOpen") return get8(c->L.open, NL, false);
-    if (f == "locInfectiousness") return get32((const uint32_t*)c->L.inf, NL);
-    if (f == "locCount") return get32(c->L.count, NL);
+    if (f == "locQuarantineUntil") return LOC_MEMBER(rec, LocRec, quarUntil);
+    if (f == "locClosedUntil") return LOC_MEMBER(rec, LocRec, closedUntil);
+    if (f == "locOpen") return LOC_MEMBER(rec, LocRec, open);
+    if (f == "locInfectiousness") return LOC_MEMBER(dyn, LocDyn, inf);
+    if (f == "locCount") return LOC_MEMBER(dyn, LocDyn, count);
+#undef AGENT_MEMBER
+#undef LOC_MEMBER
     return fail(c, PANSIM_E_ARG, "unknown field " + f);
 }
 
@@ -752,6 +1012,141 @@ int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t
     hipFree(din);
     hipFree(dout);
     return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+// ------------------------------------------------------------ replica batches
+static int batch_fail(pansim_batch* b, int code, const std::string& m) {
+    if (b) b->err = m;
+    return code;
+}
+
+void pansim_batch_destroy(pansim_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (size_t r = 0; r < b->reps.size() && r < b->ownStreams.size(); ++r) b->reps[r]->stream = b->ownStreams[r];
+    if (b->dMove) (void)hipFree(b->dMove);
+    if (b->dInf) (void)hipFree(b->dInf);
+    if (b->dTables) (void)hipFree(b->dTables);
+    if (b->hMove) (void)hipHostFree(b->hMove);
+    if (b->hInf) (void)hipHostFree(b->hInf);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* pansim_batch_error(const pansim_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_t n) {
+    if (!out || !replicas || n == 0) {
+        g_create_error = "pansim_batch_create: null argument or no replicas";
+        return PANSIM_E_ARG;
+    }
+    *out = nullptr;
+    auto* b = new pansim_batch();
+    auto bail = [&](int code, const std::string& m) {
+        g_create_error = m;
+        pansim_batch_destroy(b);
+        return code;
+    };
+    for (uint32_t r = 0; r < n; ++r) {
+        pansim_ctx* c = replicas[r];
+        if (!c) return bail(PANSIM_E_ARG, "pansim_batch_create: null replica");
+        if (c->dt != replicas[0]->dt || c->device != replicas[0]->device)
+            return bail(PANSIM_E_ARG, "pansim_batch_create: replicas differ in time step or device");
+        for (uint32_t q = 0; q < r; ++q)
+            if (replicas[q] == c) return bail(PANSIM_E_ARG, "pansim_batch_create: a replica appears twice");
+    }
+    b->dt = replicas[0]->dt;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipStreamCreate failed");
+    std::vector<uint32_t> moveRep, moveFirst(n), infRep, infFirst(n);
+    for (uint32_t r = 0; r < n; ++r) {
+        pansim_ctx* c = replicas[r];
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(PANSIM_E_DEVICE, "pansim_batch_create: sync failed");
+        b->reps.push_back(c);
+        b->ownStreams.push_back(c->stream);
+        c->stream = b->stream;
+        moveFirst[r] = (uint32_t)moveRep.size();
+        for (uint32_t t = 0; t < blocks_for(c->A.n, MOVE_TILE); ++t) moveRep.push_back(r);
+        infFirst[r] = (uint32_t)infRep.size();
+        for (uint32_t t = 0; t < blocks_for(c->A.n, INFECT_TILE); ++t) infRep.push_back(r);
+        // at rest neither pending-write counter holds anything: restart all at parity 0
+        if (hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess)
+            return bail(PANSIM_E_DEVICE, "pansim_batch_create: memset failed");
+        c->pendPar = 0;
+    }
+    b->moveBlocks = (uint32_t)moveRep.size();
+    b->infBlocks = (uint32_t)infRep.size();
+    const size_t nt = moveRep.size() + n + infRep.size() + n;
+    if (hipMalloc(&b->dMove, n * sizeof(MoveArgs)) != hipSuccess || hipMalloc(&b->dInf, n * sizeof(InfectArgs)) != hipSuccess ||
+        hipMalloc(&b->dTables, nt * sizeof(uint32_t)) != hipSuccess)
+        return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipMalloc failed");
+    if (hipHostMalloc((void**)&b->hMove, n * sizeof(MoveArgs), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->hInf, n * sizeof(InfectArgs), hipHostMallocDefault) != hipSuccess)
+        return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipHostMalloc failed");
+    std::vector<uint32_t> tables;
+    tables.insert(tables.end(), moveRep.begin(), moveRep.end());
+    tables.insert(tables.end(), moveFirst.begin(), moveFirst.end());
+    tables.insert(tables.end(), infRep.begin(), infRep.end());
+    tables.insert(tables.end(), infFirst.begin(), infFirst.end());
+    if (hipMemcpy(b->dTables, tables.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bail(PANSIM_E_DEVICE, "pansim_batch_create: table upload failed");
+    *out = b;
+    return 0;
+}
+
+int pansim_batch_set_day(pansim_batch* b, const pansim_step_params* params, const uint32_t* variant_masks, const float* k,
+                         const float* seasonality) {
+    if (!b || !params || !variant_masks || !k || !seasonality) return PANSIM_E_ARG;
+    const uint32_t n = (uint32_t)b->reps.size();
+    // the staging buffers may still feed the previous day's copy
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, "pansim_batch_set_day: sync failed");
+    for (uint32_t r = 0; r < n; ++r) {
+        pansim_ctx* c = b->reps[r];
+        if (params[r].time_step != b->dt) return batch_fail(b, PANSIM_E_ARG, "pansim_batch_set_day: time step differs");
+        b->hMove[r] = move_args(c, &params[r]);
+        b->hInf[r] = infect_args(c, params[r].timestamp, variant_masks[r], k[r], seasonality[r], b->dt);
+    }
+    if (hipMemcpyAsync(b->dMove, b->hMove, n * sizeof(MoveArgs), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->dInf, b->hInf, n * sizeof(InfectArgs), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return batch_fail(b, PANSIM_E_DEVICE, "pansim_batch_set_day: upload failed");
+    b->haveDay = true;
+    return 0;
+}
+
+int pansim_batch_step(pansim_batch* b, uint32_t timestamp) {
+    if (!b) return PANSIM_E_ARG;
+    if (!b->haveDay) return batch_fail(b, PANSIM_E_ARG, "pansim_batch_step: pansim_batch_set_day was not called");
+    const uint32_t spd = 1440u / b->dt;
+    const uint32_t stepOfDay = timestamp % spd;
+    for (pansim_ctx* c : b->reps)
+        if (timestamp - stepOfDay != c->wakeDay)
+            return batch_fail(b, PANSIM_E_ARG, "pansim_batch_step: pansim_plan_locations must run at the start of every day");
+    const uint32_t n = (uint32_t)b->reps.size();
+    BatchArgs a;
+    a.move = b->dMove;
+    a.infect = b->dInf;
+    a.moveRep = b->dTables;
+    a.moveFirst = a.moveRep + b->moveBlocks;
+    a.infRep = a.moveFirst + n;
+    a.infFirst = a.infRep + b->infBlocks;
+    a.ts = timestamp;
+    a.stepOfDay = stepOfDay;
+    a.pendPar = b->pendPar;
+    hipLaunchKernelGGL(k_move_batch, dim3(b->moveBlocks), dim3(MOVE_BLOCK), 0, b->stream, a);
+    hipLaunchKernelGGL(k_resolve_batch, dim3(n * RESOLVE_BLOCKS_PER_REPLICA), dim3(256), 0, b->stream, a);
+    hipLaunchKernelGGL(k_infect_batch, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
+    b->pendPar ^= 1u;
+    for (pansim_ctx* c : b->reps) c->pendPar = b->pendPar;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, std::string("pansim_batch_step: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int pansim_batch_sync(pansim_batch* b) {
+    if (!b) return PANSIM_E_ARG;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, "pansim_batch_sync failed");
+    return 0;
 }
 
 }  // extern "C"

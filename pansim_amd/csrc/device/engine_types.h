@@ -1,18 +1,23 @@
-// Device-side structure-of-arrays layout of one population and the launch
-// parameter blocks.  Everything is passed to kernels by value (plain
-// pointers), so a kernel never chases a host-side object.
+// Device-side layout of one population and the launch parameter blocks.
+// Everything is passed to kernels by value (plain pointers), so a kernel
+// never chases a host-side object.
 //
-// HBM layout (bytes per agent / per location), all arrays 256-byte aligned:
-//   agents: state u8, variant u8, progression u8, type u8, age u8, sex u8,
-//           precond u8, flags u8 (diagnosed, quarantined, stayedHome,
-//           noWorkAgent, healthcare worker), days i16, location u32,
-//           wake u32 (absolute step of the next full movement evaluation),
-//           lastTest u32, scalingAgeSex f32, susceptibility f32 x V,
-//           symptom scaling slot f32 x V, 12 AgentStats u32 + 4 u8 counters
-//   locations: type u8, open u8, essential u8, test flag u8, closedUntil u32,
-//           quarantineUntil u32, noWork u32, occupancy u32, infectiousness f32,
-//           infectiousness sum i64 x V (2^-32 fixed point), pending
-//           quarantine key u64
+// HBM layout, all arrays 256-byte aligned:
+//   agents, per-agent record AgentRec (32 B): the fields that the movement
+//     of one agent reads, so that a mover touches one cache line instead of
+//     one line per field;
+//   agents, structure of arrays: location u32 and wake u16 (streamed by
+//     every step; the wake step counts from the first step of the current
+//     day, since the midnight planning resets every agent's), susceptibility f32 x V (streamed by the infection pass,
+//     variant-major), and the fields only the midnight passes or rare paths
+//     touch: progression, sex, precond u8, days i16, lastTest u32,
+//     scalingAgeSex f32, symptom scaling slot f32 x V, AgentStats
+//     timestamps u32 and u8 counters, the possible-location lists (CSR);
+//   locations, LocRec (16 B): what movement reads of a location;
+//   locations, LocDyn (16 B): occupancy, exact infectiousness sum of
+//     variant 0 and infectiousness, the line that a relocation updates and
+//     the infection pass gathers; sums of variants 1.. in a separate array;
+//     pending quarantine key u64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -25,6 +30,8 @@ namespace dev {
 constexpr uint32_t NEVER = 0xFFFFFFFFu;
 constexpr uint32_t NOLOC = 0xFFFFFFFFu;
 constexpr int MAXV = 7;
+constexpr uint8_t LOC_TYPE_NONE = 0xFF;
+constexpr uint16_t WAKE_NONE = 0xFFFFu;  // no full movement evaluation left today  // type of a missing location; no location type may use it
 
 enum AgentFlag : uint8_t {
     F_DIAGNOSED = 1,
@@ -34,42 +41,73 @@ enum AgentFlag : uint8_t {
     F_HCW = 16
 };
 
+// The movement-hot fields of one agent.
+struct __align__(32) AgentRec {
+    uint32_t quarUntil;  // AgentStats::quarantinedUntilTimestamp
+    uint32_t diagTs;     // AgentStats::diagnosedTimestamp
+    uint32_t hospTs;     // AgentStats::hospitalizedTimestamp
+    uint32_t hospUntil;  // AgentStats::hospitalizedUntilTimestamp
+    uint32_t home;       // first home-type entry of the location list, NOLOC if none
+    uint32_t csrBegin;   // start of the possible-location list
+    uint8_t type;        // agent type (schedule index)
+    uint8_t state;       // progression state
+    uint8_t flags;       // AgentFlag bits
+    uint8_t variant;
+    uint8_t age;
+    uint8_t pad;
+    uint16_t csrCount;   // length of the possible-location list
+};
+static_assert(sizeof(AgentRec) == 32, "AgentRec is one 32-byte record");
+
+// What movement reads of a location.
+struct __align__(16) LocRec {
+    uint32_t quarUntil;    // quarantineUntil
+    uint32_t closedUntil;
+    uint32_t noWork;       // midnight child-care claim (0, NOWORK_FLAGGED or agent + 1)
+    uint8_t type;
+    uint8_t open;
+    uint8_t essential;
+    uint8_t flag;          // testing flag
+};
+static_assert(sizeof(LocRec) == 16, "LocRec is one 16-byte record");
+
+// Occupancy, exact variant-0 infectiousness sum (2^-32 fixed point) and
+// infectiousness of a location.
+struct __align__(16) LocDyn {
+    long long sum0;
+    uint32_t count;
+    float inf;
+};
+static_assert(sizeof(LocDyn) == 16, "LocDyn is one 16-byte record");
+
 struct Agents {
     uint32_t n;
-    uint8_t* state;
-    uint8_t* variant;
+    AgentRec* rec;
     uint8_t* prog;
-    uint8_t* type;
-    uint8_t* age;
     uint8_t* sex;
     uint8_t* precond;
-    uint8_t* flags;
     int16_t* days;
     uint32_t* location;
-    uint32_t* wake;
+    uint16_t* wake;  // step of the day of the next full movement evaluation, WAKE_NONE if none today
     uint32_t* lastTest;
     float* scal;   // scalingAgeSex
     float* susc;   // [V][n]
     float* slot6;  // [V][n] scalingSymptoms[v*7+6]
-    // AgentStats
+    // AgentStats not in AgentRec
     uint32_t* infTs;
     uint32_t* infLoc;
     uint32_t* worstTs;
     uint32_t* worstEndTs;
-    uint32_t* diagTs;
     uint32_t* quarTs;
-    uint32_t* quarUntil;
     uint32_t* daysInQ;
-    uint32_t* hospTs;
-    uint32_t* hospUntil;
     uint32_t* immTs;
     uint32_t* recTs;
     uint8_t* immCount;
+    uint8_t* immRound;  // vaccination category round, 255 = declines
     uint8_t* statVar;
     uint8_t* infCount;
     int8_t* worstState;
-    // location lists
-    const uint32_t* locOffset;
+    // possible-location lists
     const uint32_t* possLoc;
     const uint8_t* possType;
 };
@@ -77,22 +115,20 @@ struct Agents {
 struct Locs {
     uint32_t n;
     uint32_t cemetery;
-    uint8_t* type;
-    uint8_t* open;
-    uint8_t* essential;
-    uint8_t* flag;
-    uint32_t* closedUntil;
-    uint32_t* quarUntil;
-    uint32_t* noWork;
-    uint32_t* count;
-    float* inf;
-    long long* sum;                 // [V][n]
+    LocRec* rec;
+    LocDyn* dyn;
+    long long* sumX;                // [V-1][n]: sums of variants 1..V-1
     unsigned long long* pendKey;    // [n]
     const uint32_t* schools;
     const uint32_t* classrooms;
     const uint32_t* classOff;
     uint32_t nSchools;
 };
+
+// Address of the exact infectiousness sum of variant v at location l.
+__device__ __forceinline__ long long* loc_sum(const Locs& L, uint32_t v, uint32_t l) {
+    return v == 0 ? &L.dyn[l].sum0 : L.sumX + (size_t)(v - 1) * L.n + l;
+}
 
 struct Tables {
     uint32_t nStates, nVar, nProg;
@@ -115,6 +151,8 @@ struct Tables {
     const double* lenP;
     const uint32_t* evOff;
     const pansim_event* ev;
+    const int2* evWin;  // [steps per day][nOff]: active-window bounds (ab, ae) of each schedule slot
+    uint32_t nOff;      // schedule slots: agent types x 28
     pansim_location_types lt;
 };
 

@@ -49,12 +49,61 @@ __device__ __forceinline__ uint32_t wake_dec(uint32_t t, uint32_t v) {
     return w >= 0xFFFFFFFFull ? NEVER : (uint32_t)w;
 }
 
+// Location record of l; a missing location (NOLOC, the result of a lookup
+// for a type the agent has no entry for) reads as closed, never quarantined
+// and of no type.
+__device__ __forceinline__ LocRec lrec(const Locs& L, uint32_t l) {
+    if (l < L.n) return L.rec[l];
+    LocRec z;
+    z.quarUntil = 0;
+    z.closedUntil = 0;
+    z.noWork = 0;
+    z.type = LOC_TYPE_NONE;
+    z.open = 0;
+    z.essential = 0;
+    z.flag = 0;
+    return z;
+}
+
+// Active window of every schedule slot at every step of the day: the first
+// event containing the time (ab) and the first event starting after it (ae),
+// -1 when none; ae is the slot end when the last event contains the time
+// (the scan of doMovement, movementPolicies.h:470-492).
+__global__ void k_event_windows(const uint32_t* __restrict__ evOff, const pansim_event* __restrict__ ev, uint32_t nOff,
+                                uint32_t stepsPerDay, uint32_t dt, int2* __restrict__ win) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nOff * stepsPerDay) return;
+    const uint32_t step = g / nOff, off = g % nOff;
+    const TD simTime = td_from_min(step * dt);
+    const uint32_t eb = evOff[off], ee = evOff[off + 1];
+    int ab = -1, ae = -1;
+    for (uint32_t j = eb; j < ee; ++j) {
+        const pansim_event& e = ev[j];
+        bool in = !td_lt(simTime, ev_start(e)) && td_lt(simTime, ev_end(e));
+        if (in && ab == -1) ab = (int)j;
+        if (td_lt(simTime, ev_start(e))) {
+            ae = (int)j;
+            break;
+        }
+        if (j == ee - 1 && in) ae = (int)ee;
+    }
+    win[g] = make_int2(ab, ae);
+}
+
 // ------------------------------------------------ findActualLocationForType
+// The possible-location list of one agent: [b, e) in possLoc/possType, and
+// its first home-type entry.
+struct Csr {
+    uint32_t b, e, home;
+};
+__device__ __forceinline__ Csr csr_of(const AgentRec& r) { return Csr{r.csrBegin, r.csrBegin + r.csrCount, r.home}; }
+
 // movementPolicies.h:69-122; `rs` may be null when the type needs no draw.
-__device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt, uint32_t a, uint32_t ty,
+__device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt, Csr c, uint32_t ty,
                              int retry, Stream* rs) {
-    uint32_t b = A.locOffset[a], e = A.locOffset[a + 1];
-    if (ty == lt.home || ty == lt.school || ty == lt.work) {
+    if (ty == lt.home) return c.home;
+    const uint32_t b = c.b, e = c.e;
+    if (ty == lt.school || ty == lt.work) {
         for (uint32_t i = b; i < e; ++i)
             if (A.possType[i] == ty) return A.possLoc[i];
         return NOLOC;
@@ -82,7 +131,7 @@ __device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_locati
             for (uint32_t i = pos; i < e; ++i) {
                 if (A.possType[i] == ty) {
                     if (counter2 == randIdx) {
-                        if (retry && L.open[A.possLoc[i]] == 0) {
+                        if (retry && L.rec[A.possLoc[i]].open == 0) {
                             retry = 0;
                             break;
                         }
@@ -96,6 +145,95 @@ __device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_locati
     return NOLOC;
 }
 
+// The location-type bytes of one agent's list, held in registers when the
+// list has at most CSR_REG_MAX entries: entry k is byte (k & 3) of t[k >> 2].
+// The type array is padded so that the aligned word loads stay inside it.
+constexpr uint32_t CSR_REG_MAX = 16;
+struct CsrReg {
+    uint32_t b, n, home;
+    uint32_t t[4];
+};
+
+__device__ __forceinline__ CsrReg load_csr(const Agents& A, const AgentRec& r) {
+    CsrReg c;
+    c.b = r.csrBegin;
+    c.n = r.csrCount;
+    c.home = r.home;
+    c.t[0] = c.t[1] = c.t[2] = c.t[3] = 0u;
+    if (c.n - 1u < CSR_REG_MAX) {  // 1..16 entries
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(A.possType) + (c.b >> 2);
+        const uint32_t sh = c.b & 3u;
+        const uint32_t nw = (sh + c.n + 3u) >> 2;  // 1..5 words
+        uint32_t w[5];
+#pragma unroll
+        for (uint32_t j = 0; j < 5; ++j) w[j] = j < nw ? p[j] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) c.t[j] = __builtin_amdgcn_alignbyte(w[j + 1], w[j], sh);
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint32_t csr_type(const CsrReg& c, uint32_t k) {
+    // selects, not an indexed array, so that a run-time k never spills the words to memory
+    const uint32_t w = k < 8u ? (k < 4u ? c.t[0] : c.t[1]) : (k < 12u ? c.t[2] : c.t[3]);
+    return (w >> (8u * (k & 3u))) & 0xFFu;
+}
+
+// findActualLocationForType over the register copy of the types; lists longer
+// than CSR_REG_MAX take the memory version.
+__device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt,
+                                             const CsrReg& c, uint32_t ty, int retry, Stream* rs) {
+    if (ty == lt.home) return c.home;
+    if (c.n > CSR_REG_MAX) return find_loc(A, L, lt, Csr{c.b, c.b + c.n, c.home}, ty, retry, rs);
+    if (ty == lt.school || ty == lt.work) {
+#pragma unroll
+        for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
+            if (k < c.n && csr_type(c, k) == ty) return A.possLoc[c.b + k];
+        return NOLOC;
+    }
+    uint32_t counter = 0, pos = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
+        if (k < c.n && csr_type(c, k) == ty) {
+            if (counter == 0) pos = k;
+            ++counter;
+        }
+    if (counter == 0) return NOLOC;
+    if (counter == 1) return A.possLoc[c.b + pos];
+    uint32_t counter2 = counter, randIdx = counter;
+    int redraws = 0;
+    for (;;) {
+        while (randIdx == counter2) {
+            if (redraws == 65) {
+                randIdx = (counter2 + 1) % counter;
+                break;
+            }
+            randIdx = rs->uint_below(counter);
+            ++redraws;
+        }
+        uint32_t sel = 0, seen = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
+            if (k < c.n && csr_type(c, k) == ty) {
+                if (seen == randIdx) sel = k;
+                ++seen;
+            }
+        const uint32_t l = A.possLoc[c.b + sel];
+        if (retry && L.rec[l].open == 0) {  // closed: draw another one, once
+            retry = 0;
+            counter2 = randIdx;
+            continue;
+        }
+        return l;
+    }
+}
+
+// The same for agent a, reading its record.
+__device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt, uint32_t a,
+                                             uint32_t ty, int retry, Stream* rs) {
+    return find_loc(A, L, lt, csr_of(A.rec[a]), ty, retry, rs);
+}
+
 struct MoveArgs {
     Agents A;
     Locs L;
@@ -105,6 +243,7 @@ struct MoveArgs {
     uint64_t seed;
     uint32_t replica;
     uint32_t curfewStep;  // all agents take the full path this step
+    uint32_t stepOfDay;   // timestamp % steps per day
     uint32_t nEvents;
     uint32_t pendPar;  // parity of the pending-write counter used this step
     unsigned long long* counters;  // MOVE_CTR_SLOTS pairs (evaluated, relocated)
@@ -114,6 +253,7 @@ constexpr uint32_t MOVE_BLOCK = 256;
 constexpr uint32_t MOVE_ITEMS = 16;
 constexpr uint32_t MOVE_TILE = MOVE_BLOCK * MOVE_ITEMS;  // agents per block
 constexpr uint32_t MOVE_CTR_SLOTS = 64;
+static_assert(MOVE_BLOCK == 256 && MOVE_ITEMS == 16, "the wake scan loads 2 x 8 wake steps per thread");
 
 __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, uint32_t loc, uint32_t until) {
     unsigned long long key = ((unsigned long long)agent << 32) | (unsigned long long)until;
@@ -124,34 +264,25 @@ __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, ui
     }
 }
 
-// movementPolicies.h:171-245 with the location writes deferred to the end of the step
-__device__ void quarantine_agent(const MoveArgs& m, uint32_t i, uint32_t until, uint8_t& fl, Stream& rs) {
-    const Agents& A = m.A;
-    const Locs& L = m.L;
+// The location part of quarantineAgent (movementPolicies.h:202-241) for a
+// diagnosed agent: the home, and with quarantinePolicy > 2 the classroom and
+// workplace, get pending quarantine writes (applied at the end of the step).
+// It runs once, after the rest of the agent's movement routine, so its
+// classroom draw is the routine's last draw.
+__device__ __forceinline__ void quarantine_locations(const MoveArgs& m, uint32_t i, const CsrReg& c, uint32_t until,
+                                                     uint32_t prev, Stream& rs) {
     const pansim_location_types& lt = m.T.lt;
-    uint32_t ts = m.p.timestamp;
-    if (m.p.quarantine_policy == 0) return;
-    bool infected = m.T.stInfected[A.state[i]] != 0;
-    if (A.diagTs[i] > 0 && !infected && m.p.quarantine_immune_active == 0) return;
-    if (!(fl & F_DIAGNOSED) && A.immTs[i] > 0 && m.p.quarantine_immune_active == 0) return;
-    fl |= F_QUARANTINED;
-    A.quarTs[i] = ts;
-    uint32_t prev = A.quarUntil[i];
-    A.quarUntil[i] = until;
-    A.daysInQ[i] += (until - ts) / (24u * 60u / m.p.time_step);
-    if (fl & F_DIAGNOSED) {
-        uint32_t home = find_loc(A, L, lt, i, lt.home, 0, nullptr);
-        if (m.p.quarantine_policy > 1 && home != NOLOC) pend_write(m, i, home, until);
-        if (m.p.quarantine_policy > 2 && prev < ts) {
-            uint32_t cls = find_loc(A, L, lt, i, lt.classroom, 0, &rs);
-            uint32_t work = find_loc(A, L, lt, i, lt.work, 0, nullptr);
-            uint32_t toClose[2] = {cls, work};
-            for (int k = 0; k < 2; ++k) {
-                uint32_t loc = toClose[k];
-                if (loc != NOLOC && (L.type[loc] == lt.work || L.type[loc] == lt.classroom) && L.quarUntil[loc] < ts &&
-                    L.open[loc] && L.closedUntil[loc] < ts)
-                    pend_write(m, i, loc, until);
-            }
+    const uint32_t ts = m.p.timestamp;
+    if (m.p.quarantine_policy > 1 && c.home != NOLOC) pend_write(m, i, c.home, until);
+    if (m.p.quarantine_policy > 2 && prev < ts) {
+        uint32_t toClose[2] = {find_loc(m.A, m.L, lt, c, lt.classroom, 0, &rs), find_loc(m.A, m.L, lt, c, lt.work, 0, nullptr)};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t loc = toClose[k];
+            if (loc == NOLOC) continue;
+            const LocRec lr = m.L.rec[loc];
+            if ((lr.type == lt.work || lr.type == lt.classroom) && lr.quarUntil < ts && lr.open && lr.closedUntil < ts)
+                pend_write(m, i, loc, until);
         }
     }
 }
@@ -160,7 +291,11 @@ __device__ void quarantine_agent(const MoveArgs& m, uint32_t i, uint32_t until, 
 // Writes location, wake step, flags and stats of agent i; keeps the
 // per-location occupancy and infectiousness sums up to date.  Returns
 // whether the agent changed location.
-__device__ bool do_movement(const MoveArgs& m, uint32_t i) {
+__device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
+    // one 32-byte load: every agent field the routine reads
+    const AgentRec r0 = m.A.rec[i];
+    AgentRec r = r0;
+    const CsrReg csr = load_csr(m.A, r0);
     const Agents& A = m.A;
     const Locs& L = m.L;
     const Tables& T = m.T;
@@ -171,34 +306,57 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
     const TD simTime = td_from_min(minutes);
     const uint32_t stepsUntilMidnight = (1440u - minutes) / dt;
     const uint32_t qLen = m.p.quarantine_length * 24u * 60u / dt;
-    const uint32_t agentType = A.type[i];
+    const uint32_t agentType = r.type;
     const uint32_t oldLoc = A.location[i];
     uint32_t loc = oldLoc;
-    uint8_t fl = A.flags[i];
-    const uint8_t fl0 = fl;
-    uint32_t wake = A.wake[i];
+    const uint32_t dayStart = ts - minutes / dt;
+    const uint16_t wake0 = A.wake[i];
+    uint32_t wake = wake0 == WAKE_NONE ? NEVER : dayStart + wake0;
     Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
-    const uint8_t st = A.state[i];
+    const uint8_t st = r.state;
     const uint8_t w = T.stWB[st];
+
+    // quarantineAgent (movementPolicies.h:171-245) without its location
+    // writes: an agent is quarantined at most once per routine, and only a
+    // diagnosed one leaves location writes, done after the routine
+    bool qLoc = false;
+    uint32_t qLocUntil = 0, qLocPrev = 0;
+    auto quarantine = [&](uint32_t until) {
+        if (m.p.quarantine_policy == 0) return;
+        const bool infected = T.stInfected[r.state] != 0;
+        if (r.diagTs > 0 && !infected && m.p.quarantine_immune_active == 0) return;
+        if (!(r.flags & F_DIAGNOSED) && A.immTs[i] > 0 && m.p.quarantine_immune_active == 0) return;
+        r.flags |= F_QUARANTINED;
+        A.quarTs[i] = ts;
+        const uint32_t prev = r.quarUntil;
+        r.quarUntil = until;
+        A.daysInQ[i] += (until - ts) / (24u * 60u / dt);
+        if (r.flags & F_DIAGNOSED) {
+            qLoc = true;
+            qLocUntil = until;
+            qLocPrev = prev;
+        }
+    };
 
     bool done = false;
     // curfew (movementPolicies.h:267-326)
     if (m.p.enable_curfew && m.p.curfew_begin == minutes / dt) {
         bool deadOrHosp = (w == 4 || w == 3);
-        bool hospNonCovid = A.hospTs[i] <= ts && A.hospUntil[i] > ts;
+        bool hospNonCovid = r.hospTs <= ts && r.hospUntil > ts;
         if (!deadOrHosp && !hospNonCovid) {
             if (agentType + 1 == 7) {
-                uint32_t workplace = find_loc(A, L, lt, i, lt.work, 0, nullptr);
-                if (loc != workplace || L.open[workplace] == 0) {
-                    loc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
-                    if (L.open[workplace] == 0) {
+                uint32_t workplace = find_loc(A, L, lt, csr, lt.work, 0, nullptr);
+                const bool workOpen = lrec(L, workplace).open != 0;
+                if (loc != workplace || !workOpen) {
+                    loc = csr.home;
+                    if (!workOpen) {
                         wake = wake_keep(ts, stepsUntilMidnight);
                         done = true;
                     }
                 }
             } else {
                 wake = wake_keep(ts, stepsUntilMidnight);
-                loc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+                loc = csr.home;
                 done = true;
             }
         }
@@ -206,64 +364,66 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
     if (!done && wake != ts) done = true;  // countdown > 0: nothing else happens this step
 
     if (!done) do {
-        if (A.quarUntil[i] <= ts) fl &= (uint8_t)~F_QUARANTINED;
+        if (r.quarUntil <= ts) r.flags &= (uint8_t)~F_QUARANTINED;
         if (w == 4) {  // dead
             wake = NEVER;
             loc = L.cemetery;
             break;
         }
-        const uint32_t agentHome = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+        const uint32_t agentHome = csr.home;
         uint32_t sum = 0;
         auto diagnose = [&]() {
-            if (!(fl & F_DIAGNOSED) && T.stInf[st] * T.varMul[A.variant[i]] > 0.0f) {
-                fl |= F_DIAGNOSED;
+            if (!(r.flags & F_DIAGNOSED) && T.stInf[st] * T.varMul[r.variant] > 0.0f) {
+                r.flags |= F_DIAGNOSED;
                 uint32_t d = ts;
                 if (stepsUntilMidnight == 24u * 60u / dt) d++;
-                A.diagTs[i] = d;
-                quarantine_agent(m, i, ts + qLen, fl, rs);
+                r.diagTs = d;
+                quarantine(ts + qLen);
             }
         };
-        if (A.hospTs[i] <= ts && A.hospUntil[i] > ts && w != 3 && w != 4) {
-            uint32_t v = A.hospUntil[i] - ts - 1;
+        if (r.hospTs <= ts && r.hospUntil > ts && w != 3 && w != 4) {
+            uint32_t v = r.hospUntil - ts - 1;
             sum = v < stepsUntilMidnight ? v : stepsUntilMidnight;
-            loc = find_loc(A, L, lt, i, lt.hospital, 0, &rs);
+            loc = find_loc(A, L, lt, csr, lt.hospital, 0, &rs);
             diagnose();
             wake = wake_keep(ts, sum);
             break;
         }
         if (w == 3) {
             sum = stepsUntilMidnight;
-            loc = find_loc(A, L, lt, i, lt.hospital, 0, &rs);
+            loc = find_loc(A, L, lt, csr, lt.hospital, 0, &rs);
             diagnose();
             wake = wake_keep(ts, sum);
             break;
         }
-        auto quarantinable = [&](uint32_t l) {
-            uint32_t t = L.type[l];
+        auto quarantinable = [&](const LocRec& lr) {
+            uint32_t t = lr.type;
             return t == lt.home || t == lt.school || t == lt.classroom || t == lt.work;
         };
         auto stayedHomeCheck = [&]() {
-            if (agentHome != NOLOC && loc != agentHome && sum > 60u / dt) fl &= (uint8_t)~F_STAYED_HOME;
+            if (agentHome != NOLOC && loc != agentHome && sum > 60u / dt) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
-        if (m.p.quarantine_policy > 1 && ts < L.quarUntil[loc] && quarantinable(loc)) {
-            if (!(fl & F_QUARANTINED)) quarantine_agent(m, i, L.quarUntil[loc], fl, rs);
-            if (fl & F_QUARANTINED) {
-                uint32_t v = L.quarUntil[loc] - ts - 1;
-                sum = v < stepsUntilMidnight ? v : stepsUntilMidnight;
-                uint32_t homeLocation = find_loc(A, L, lt, i, lt.home, 0, nullptr);
-                if (homeLocation != loc) loc = homeLocation;
-                stayedHomeCheck();
-                wake = wake_keep(ts, sum);
-                break;
+        if (m.p.quarantine_policy > 1) {
+            const LocRec lr = lrec(L, loc);
+            if (ts < lr.quarUntil && quarantinable(lr)) {
+                if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
+                if (r.flags & F_QUARANTINED) {
+                    uint32_t v = lr.quarUntil - ts - 1;
+                    sum = v < stepsUntilMidnight ? v : stepsUntilMidnight;
+                    if (csr.home != loc) loc = csr.home;
+                    stayedHomeCheck();
+                    wake = wake_keep(ts, sum);
+                    break;
+                }
             }
         }
         {
             bool qp = m.p.quarantine_policy > 0;
-            uint32_t dts = A.diagTs[i];
-            if ((qp && ((fl & F_DIAGNOSED) || (dts > 0 && (ts - dts) < qLen))) || (qp && (fl & F_QUARANTINED))) {
-                if (qp && !(fl & F_QUARANTINED)) quarantine_agent(m, i, ts + qLen, fl, rs);
-                if (fl & F_QUARANTINED) {
-                    loc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+            uint32_t dts = r.diagTs;
+            if ((qp && ((r.flags & F_DIAGNOSED) || (dts > 0 && (ts - dts) < qLen))) || (qp && (r.flags & F_QUARANTINED))) {
+                if (qp && !(r.flags & F_QUARANTINED)) quarantine(ts + qLen);
+                if (r.flags & F_QUARANTINED) {
+                    loc = csr.home;
                     sum = stepsUntilMidnight;
                     stayedHomeCheck();
                     wake = wake_keep(ts, sum);
@@ -271,24 +431,14 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 }
             }
         }
-        // schedule window (agentType.cpp:13-15)
-        uint32_t off = 7u * w + m.p.day_of_week + agentType * 28u;
-        uint32_t eb = T.evOff[off], ee = T.evOff[off + 1];
-        int ab = -1, ae = -1;
-        for (uint32_t j = eb; j < ee; ++j) {
-            const pansim_event& e = T.ev[j];
-            bool in = !td_lt(simTime, ev_start(e)) && td_lt(simTime, ev_end(e));
-            if (in && ab == -1) ab = (int)j;
-            if (td_lt(simTime, ev_start(e))) {
-                ae = (int)j;
-                break;
-            }
-            if (j == ee - 1 && in) ae = (int)ee;
-        }
+        // schedule window (agentType.cpp:13-15), precomputed per step of the day
+        const uint32_t off = 7u * w + m.p.day_of_week + agentType * 28u;
+        const int2 win = T.evWin[(size_t)(minutes / dt) * T.nOff + off];
+        const int ab = win.x, ae = win.y;
         uint32_t newType = NOLOC;
         if (ab == -1 && ae == -1) {  // Case 1
             newType = (w == 3) ? lt.hospital : lt.home;
-            loc = find_loc(A, L, lt, i, newType, 0, &rs);
+            loc = find_loc(A, L, lt, csr, newType, 0, &rs);
             sum = stepsUntilMidnight;
         }
         if (ab != -1) {  // Cases 2 and 4
@@ -311,16 +461,17 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 newType = T.ev[ab + idx].loc_type;
                 dur = ev_dur(T.ev[ab + idx]);
             }
-            if ((fl & F_NO_WORK) && newType == lt.work) {
+            if ((r.flags & F_NO_WORK) && newType == lt.work) {
                 newType = lt.home;
                 nPot = 1;
-                if (!(fl & F_QUARANTINED)) A.daysInQ[i] += 1;
+                if (!(r.flags & F_QUARANTINED)) A.daysInQ[i] += 1;
             }
-            uint32_t newLoc = find_loc(A, L, lt, i, newType, nPot == 1, &rs);
-            bool tooOld = (newType == lt.school || newType == lt.classroom) && A.age[i] >= m.p.school_age_restriction;
+            uint32_t newLoc = find_loc(A, L, lt, csr, newType, nPot == 1, &rs);
+            bool tooOld = (newType == lt.school || newType == lt.classroom) && r.age >= m.p.school_age_restriction;
             bool lockdown = m.p.lockdown_nonvacc_active && A.immTs[i] == 0;
             auto closedHere = [&](uint32_t l, uint32_t t) {
-                return (L.open[l] == 0 || L.closedUntil[l] > ts) && t != lt.work;
+                const LocRec lr = lrec(L, l);
+                return (lr.open == 0 || lr.closedUntil > ts) && t != lt.work;
             };
             if (tooOld || closedHere(newLoc, newType) || lockdown) {
                 if (nPot > 1) {
@@ -335,14 +486,14 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
                     if (idx == nPot) idx--;
                     newType = T.ev[ab + idx].loc_type;
                     dur = ev_dur(T.ev[ab + idx]);
-                    newLoc = find_loc(A, L, lt, i, newType, 0, &rs);
-                    if (tooOld || closedHere(newLoc, newType) || lockdown) newLoc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+                    newLoc = find_loc(A, L, lt, csr, newType, 0, &rs);
+                    if (tooOld || closedHere(newLoc, newType) || lockdown) newLoc = csr.home;
                 } else {
-                    newLoc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+                    newLoc = csr.home;
                 }
             } else if (newType == lt.school) {
-                uint32_t cls = find_loc(A, L, lt, i, lt.classroom, 0, &rs);
-                if (cls != NOLOC && L.closedUntil[cls] > ts) newLoc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+                uint32_t cls = find_loc(A, L, lt, csr, lt.classroom, 0, &rs);
+                if (cls != NOLOC && L.rec[cls].closedUntil > ts) newLoc = csr.home;
             }
             loc = newLoc;
             if (dur.h > 24) {
@@ -371,7 +522,8 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
             }
             if (agentType + 1 == 7 && m.p.enable_curfew &&
                 (m.p.curfew_begin <= minutes / dt || m.p.curfew_end > minutes / dt)) {
-                bool workButClosed = newType == lt.work && (L.open[newLoc] == 0 || L.closedUntil[newLoc] > ts);
+                const LocRec lr = lrec(L, newLoc);
+                bool workButClosed = newType == lt.work && (lr.open == 0 || lr.closedUntil > ts);
                 if (workButClosed || (newType != lt.work && newType != lt.home)) loc = agentHome;
             }
         }
@@ -383,22 +535,24 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
             if (sum < 30u / dt) {
             } else if (sum < 60u / dt) {
                 newType = lt.public_space;
-                loc = find_loc(A, L, lt, i, lt.public_space, 0, &rs);
+                loc = find_loc(A, L, lt, csr, lt.public_space, 0, &rs);
             } else {
                 newType = lt.home;
-                loc = find_loc(A, L, lt, i, lt.home, 0, nullptr);
+                loc = csr.home;
             }
         }
-        if (m.p.quarantine_policy > 1 && ts < L.quarUntil[loc] && quarantinable(loc)) {
-            if (!(fl & F_QUARANTINED)) quarantine_agent(m, i, L.quarUntil[loc], fl, rs);
-            if (fl & F_QUARANTINED) {
-                uint32_t v = L.quarUntil[loc] - ts - 1;
-                sum = v < stepsUntilMidnight ? v : stepsUntilMidnight;
-                uint32_t homeLocation = find_loc(A, L, lt, i, lt.home, 0, nullptr);
-                if (homeLocation != loc) loc = homeLocation;
-                stayedHomeCheck();
-                wake = wake_keep(ts, sum);
-                break;
+        if (m.p.quarantine_policy > 1) {
+            const LocRec lr = lrec(L, loc);
+            if (ts < lr.quarUntil && quarantinable(lr)) {
+                if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
+                if (r.flags & F_QUARANTINED) {
+                    uint32_t v = lr.quarUntil - ts - 1;
+                    sum = v < stepsUntilMidnight ? v : stepsUntilMidnight;
+                    if (csr.home != loc) loc = csr.home;
+                    stayedHomeCheck();
+                    wake = wake_keep(ts, sum);
+                    break;
+                }
             }
         }
         if (newType == lt.hospital || (newType == lt.doctor && rs.real(1.0) < 0.7)) diagnose();
@@ -406,19 +560,26 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
         wake = wake_dec(ts, sum);
     } while (false);
 
-    if (fl != fl0) A.flags[i] = fl;
-    if (wake != A.wake[i]) A.wake[i] = wake;
+    if (qLoc) quarantine_locations(m, i, csr, qLocUntil, qLocPrev, rs);
+    if (r.flags != r0.flags) A.rec[i].flags = r.flags;
+    if (r.diagTs != r0.diagTs) A.rec[i].diagTs = r.diagTs;
+    if (r.quarUntil != r0.quarUntil) A.rec[i].quarUntil = r.quarUntil;
+    // a wake step beyond today is never reached: the midnight planning resets it
+    const uint16_t wake16 = (wake == NEVER || wake - dayStart >= 1440u / dt) ? WAKE_NONE : (uint16_t)(wake - dayStart);
+    if (wake16 != wake0) A.wake[i] = wake16;
+    // a location type the agent has no entry for leaves it where it is
+    if (loc == NOLOC) loc = oldLoc;
     if (loc == oldLoc) return false;
     A.location[i] = loc;
-    atomicSub(&L.count[oldLoc], 1u);
-    atomicAdd(&L.count[loc], 1u);
-    float x = T.stInf[st] * T.varMul[A.variant[i]];
+    const uint8_t var = r.variant;
+    float x = T.stInf[st] * T.varMul[var];
     if (x != 0.0f) {
         long long q = fixed32(x);
-        long long* s = L.sum + (size_t)A.variant[i] * L.n;
-        atomicAdd((unsigned long long*)&s[oldLoc], (unsigned long long)(-q));
-        atomicAdd((unsigned long long*)&s[loc], (unsigned long long)q);
+        atomicAdd((unsigned long long*)loc_sum(L, var, oldLoc), (unsigned long long)(-q));
+        atomicAdd((unsigned long long*)loc_sum(L, var, loc), (unsigned long long)q);
     }
+    atomicSub(&L.dyn[oldLoc].count, 1u);
+    atomicAdd(&L.dyn[loc].count, 1u);
     return true;
 }
 
@@ -429,36 +590,40 @@ __device__ bool do_movement(const MoveArgs& m, uint32_t i) {
 // per-agent writes are private, the per-location sums are integer atomics and
 // the location quarantine writes are deferred to k_resolve_pending.
 // Evaluated / relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
-__global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) {
+__device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t tile) {
     __shared__ uint32_t list[MOVE_TILE];
     __shared__ uint32_t nList, nMoved;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
-    const uint32_t base = blockIdx.x * MOVE_TILE;
+    const uint32_t base = tile * MOVE_TILE;
     const uint32_t n = m.A.n;
-    const uint32_t ts = m.p.timestamp;
     if (tid == 0) {
         nList = 0;
         nMoved = 0;
     }
-    uint32_t w[MOVE_ITEMS];
+    // two 16-byte loads per thread: agents base + k * (MOVE_TILE / 2) + tid * 8 + j
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u* wv = reinterpret_cast<const v4u*>(m.A.wake + base);
+    v4u w[2];
 #pragma unroll
-    for (uint32_t k = 0; k < MOVE_ITEMS; ++k) {
-        uint32_t i = base + k * MOVE_BLOCK + tid;
-        w[k] = i < n ? __builtin_nontemporal_load(&m.A.wake[i]) : ts + 1u;
-    }
+    for (uint32_t k = 0; k < 2; ++k) w[k] = __builtin_nontemporal_load(&wv[k * (MOVE_TILE / 16) + tid]);
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (uint32_t k = 0; k < MOVE_ITEMS; ++k) {
-        uint32_t i = base + k * MOVE_BLOCK + tid;
-        bool mover = i < n && (m.curfewStep || w[k] == ts);
-        unsigned long long mask = __ballot(mover);
-        if (mask != 0ull) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
-            pos = __shfl(pos, 0, 64);
-            if (mover) list[pos + (uint32_t)__popcll(mask & below)] = i;
+    for (uint32_t k = 0; k < 2; ++k) {
+        const uint32_t words[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t i = base + k * (MOVE_TILE / 2) + tid * 8u + j;
+            const uint32_t w16 = (words[j >> 1] >> (16u * (j & 1u))) & 0xFFFFu;
+            bool mover = i < n && (m.curfewStep || w16 == m.stepOfDay);
+            unsigned long long mask = __ballot(mover);
+            if (mask != 0ull) {
+                uint32_t pos = 0;
+                if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
+                pos = __shfl(pos, 0, 64);
+                if (mover) list[pos + (uint32_t)__popcll(mask & below)] = i;
+            }
         }
     }
     __syncthreads();
@@ -468,24 +633,32 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) {
     if (moved) atomicAdd(&nMoved, moved);
     __syncthreads();
     if (tid == 0 && cnt) {
-        unsigned long long* c = m.counters + 2u * (blockIdx.x % MOVE_CTR_SLOTS);
+        unsigned long long* c = m.counters + 2u * (tile % MOVE_CTR_SLOTS);
         atomicAdd(&c[0], (unsigned long long)cnt);
         if (nMoved) atomicAdd(&c[1], (unsigned long long)nMoved);
     }
 }
 
+__global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) { move_tile(m, blockIdx.x); }
+
 // End of the step's movement: the deferred quarantine writes; the highest
 // writing agent index of each location wins.  The step's pending list count
 // is pendCount[par]; the other parity's count is cleared for the next step.
-__global__ void __launch_bounds__(256) k_resolve_pending(Locs L, StepScratch S, uint32_t par) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) S.pendCount[par ^ 1u] = 0u;
+// Block `blk` of `nBlk` blocks working on this population.
+__device__ __forceinline__ void resolve_pending(const Locs& L, const StepScratch& S, uint32_t par, uint32_t blk,
+                                                uint32_t nBlk) {
+    if (blk == 0 && threadIdx.x == 0) S.pendCount[par ^ 1u] = 0u;
     uint32_t pc = S.pendCount[par];
     if (pc > S.pendCap) pc = S.pendCap;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < pc; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < pc; j += nBlk * blockDim.x) {
         uint32_t loc = S.pendList[j];
         unsigned long long key = atomicExch(&L.pendKey[loc], 0ull);
-        if (key != 0ull) L.quarUntil[loc] = (uint32_t)key;
+        if (key != 0ull) L.rec[loc].quarUntil = (uint32_t)key;
     }
+}
+
+__global__ void __launch_bounds__(256) k_resolve_pending(Locs L, StepScratch S, uint32_t par) {
+    resolve_pending(L, S, par, blockIdx.x, gridDim.x);
 }
 
 struct InfectArgs {
@@ -508,22 +681,22 @@ struct InfectArgs {
 // agents (stride INFECT_BLOCK) and issues their independent loads together,
 // so that a wave keeps several memory round trips in flight.
 constexpr uint32_t INFECT_BLOCK = 256;
-constexpr uint32_t INFECT_ITEMS = 4;
+constexpr uint32_t INFECT_ITEMS = 8;
 constexpr uint32_t INFECT_TILE = INFECT_BLOCK * INFECT_ITEMS;
 
 __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
-                                             float s) {
+                                             uint32_t count, float inf, float s) {
     const uint32_t n = a.A.n;
     float num = (float)((double)e * (1.0 / 4294967296.0));
-    float density = num / (float)a.L.count[loc];
-    float ratio = 1.0f - ps_expf(-a.k * density * a.L.inf[loc] * a.season * a.dt);
+    float density = num / (float)count;
+    float ratio = 1.0f - ps_expf(-a.k * density * inf * a.season * a.dt);
     if (ratio == 0.0f) return;
     Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
     if (rs.unit() < (double)(ratio * s)) {
         // gotInfected (dynamicPPState.cpp:235-244) + stats (locationList.h:286-291)
         uint8_t fi = (uint8_t)a.T.firstInfected;
-        a.A.state[i] = fi;
-        a.A.variant[i] = (uint8_t)v;
+        a.A.rec[i].state = fi;
+        a.A.rec[i].variant = (uint8_t)v;
         a.A.days[i] = -2;
         float sf = a.T.stSusc[fi] ? 1.0f : 0.0f;
         for (uint32_t u = 0; u < a.T.nVar; ++u) a.A.susc[(size_t)u * n + i] = sf;
@@ -536,10 +709,10 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
     }
 }
 
-__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
+__device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) {
     if (a.mask == 0u) return;
     const uint32_t n = a.A.n;
-    const uint32_t base = blockIdx.x * INFECT_TILE + threadIdx.x;
+    const uint32_t base = tile * INFECT_TILE + threadIdx.x;
     uint32_t loc[INFECT_ITEMS];
 #pragma unroll
     for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
@@ -556,13 +729,67 @@ __global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
             uint32_t i = base + k * INFECT_BLOCK;
             s[k] = i < n ? a.A.susc[(size_t)v * n + i] : 0.0f;
         }
-        long long e[INFECT_ITEMS];
-#pragma unroll
-        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) e[k] = s[k] > 0.0f ? a.L.sum[(size_t)v * a.L.n + loc[k]] : 0ll;
+        // one 8-byte gather per susceptible agent: the location's sum of this variant
+        const long long* sv = v == 0 ? &a.L.dyn[0].sum0 : a.L.sumX + (size_t)(v - 1) * a.L.n;
+        const uint32_t pitch = v == 0 ? (uint32_t)(sizeof(LocDyn) / sizeof(long long)) : 1u;
+        uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
-            if (e[k] != 0) infect_agent(a, base + k * INFECT_BLOCK, v, loc[k], e[k], s[k]);
+            if (s[k] > 0.0f && sv[(size_t)loc[k] * pitch] != 0) hits |= 1u << k;
+        // the rare agents at a location with infectious visitors: one at a time
+        while (hits) {
+            const uint32_t k = (uint32_t)__builtin_ctz(hits);
+            hits &= hits - 1u;
+            const uint32_t i = base + k * INFECT_BLOCK;
+            const uint32_t l = a.A.location[i];
+            const LocDyn d = a.L.dyn[l];
+            infect_agent(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, a.A.susc[(size_t)v * n + i]);
+        }
     }
+}
+
+__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) { infect_tile(a, blockIdx.x); }
+
+// ------------------------------------------------------------ replica batches
+// Many populations (replicas) stepped by one launch per kernel: every block
+// finds its replica in a per-block table and runs the same tile code on that
+// replica's arrays.  The per-day parameters of every replica live in device
+// arrays; the step's timestamp is a kernel argument, shared by all replicas.
+constexpr uint32_t RESOLVE_BLOCKS_PER_REPLICA = 4;
+
+struct BatchArgs {
+    const MoveArgs* move;       // [replicas]
+    const InfectArgs* infect;   // [replicas]
+    const uint32_t* moveRep;    // [movement blocks]: replica of each block
+    const uint32_t* moveFirst;  // [replicas]: its first movement block
+    const uint32_t* infRep;     // [infection blocks]
+    const uint32_t* infFirst;   // [replicas]
+    uint32_t ts;
+    uint32_t stepOfDay;
+    uint32_t pendPar;
+};
+
+__global__ void __launch_bounds__(MOVE_BLOCK) k_move_batch(BatchArgs b) {
+    const uint32_t r = b.moveRep[blockIdx.x];
+    MoveArgs m = b.move[r];
+    m.p.timestamp = b.ts;
+    m.stepOfDay = b.stepOfDay;
+    m.pendPar = b.pendPar;
+    m.curfewStep = (m.p.enable_curfew && m.p.curfew_begin == b.stepOfDay) ? 1u : 0u;
+    move_tile(m, blockIdx.x - b.moveFirst[r]);
+}
+
+__global__ void __launch_bounds__(256) k_resolve_batch(BatchArgs b) {
+    const uint32_t r = blockIdx.x / RESOLVE_BLOCKS_PER_REPLICA;
+    const MoveArgs& m = b.move[r];
+    resolve_pending(m.L, m.S, b.pendPar, blockIdx.x % RESOLVE_BLOCKS_PER_REPLICA, RESOLVE_BLOCKS_PER_REPLICA);
+}
+
+__global__ void __launch_bounds__(INFECT_BLOCK) k_infect_batch(BatchArgs b) {
+    const uint32_t r = b.infRep[blockIdx.x];
+    InfectArgs a = b.infect[r];
+    a.timestamp = b.ts;
+    infect_tile(a, blockIdx.x - b.infFirst[r]);
 }
 
 }  // namespace dev

@@ -361,6 +361,46 @@ Scenario build_scenario(const Options& opt) {
     if (sc.progressionScaling.size() < sc.variantMultiplier.size() || sc.deathScaling.size() < sc.variantMultiplier.size())
         throw InitError(ERR_BAD_ARG, "diseaseProgressionScaling/diseaseProgressionDeathScaling need one value per variant");
     sc.dOffset = split_int(opt.d_offset, ',');
+    {  // Immunization::initializeArgs (immunization.h:139-222)
+        VaccinationPlan& v = sc.vacc;
+        v.startDay = opt.immunizationStart;
+        v.boosterStart = split_int(opt.boosterStart, ',');
+        auto read_all = [&](const std::string& path) {
+            std::ifstream t(resolve_path(opt, path));
+            std::stringstream buf;
+            buf << t.rdbuf();
+            return buf.str();
+        };
+        char* end = nullptr;
+        v.dailyDoses = (unsigned)(std::strtoul(opt.immunizationsPerWeek.c_str(), &end, 10) / 7);
+        if (end != opt.immunizationsPerWeek.c_str() + opt.immunizationsPerWeek.size())
+            v.vaccPerWeek = split_float(read_all(opt.immunizationsPerWeek), ',');
+        const size_t rounds = v.boosterStart.size();
+        v.boosterPerWeek.assign(rounds, {});
+        v.boosterAgePercent.assign(rounds, {});
+        v.dailyBoosters = (unsigned)(std::strtoul(opt.boostersPerWeek.c_str(), &end, 10) / 7);
+        if (end != opt.boostersPerWeek.c_str() + opt.boostersPerWeek.size()) {
+            std::ifstream t(resolve_path(opt, opt.boostersPerWeek));
+            std::string line;
+            size_t r = 0;
+            while (r < rounds && std::getline(t, line)) {
+                if (line.empty() || line[0] == '#') continue;
+                v.boosterPerWeek[r] = split_float(line, ',');
+                if (std::getline(t, line)) v.boosterAgePercent[r] = split_float(line, ',');
+                ++r;
+            }
+            if (r != rounds) throw InitError(ERR_BAD_ARG, "booster rounds size mismatch");
+        }
+        for (size_t r = 0; r < rounds; ++r)
+            if (v.boosterAgePercent[r].size() < 10)
+                throw InitError(ERR_BAD_ARG, "boostersPerWeek: every booster round needs 10 age-group percentages");
+        v.order = split_int(opt.immunizationOrder, ',');
+        if (v.order.size() != 10) throw InitError(ERR_BAD_ARG, "immunizationOrder mush have exactly 10 values");
+        for (int x : v.order)
+            if (x > 10) throw InitError(ERR_BAD_ARG, "immunizationOrder values have to be  less or equal to 10");
+        v.groupLevel = split_float(opt.vaccinationGroupLevel, ',');
+        if (v.groupLevel.size() < 10) throw InitError(ERR_BAD_ARG, "vaccinationGroupLevel needs 10 values");
+    }
     if (opt.testingMethod != "PCR" && opt.testingMethod != "antigen")
         throw InitError(ERR_BAD_TESTING,
                         "unrecognized testingMethod " + opt.testingMethod + " must be either PCR or antigen");

@@ -219,8 +219,23 @@ struct ClosureRuleRaw {
     std::vector<int> locationTypesToClose;
 };
 
+// Vaccination and booster rollout inputs (Immunization::initializeArgs,
+// include/immunization.h:139-222).
+struct VaccinationPlan {
+    unsigned startDay = 96;                              // immunizationStart
+    std::vector<int> boosterStart;                       // first day of each booster round
+    unsigned dailyDoses = 0;                             // immunizationsPerWeek given as a number, / 7
+    std::vector<float> vaccPerWeek;                      // or per week, from a file
+    unsigned dailyBoosters = 0;                          // boostersPerWeek given as a number, / 7
+    std::vector<std::vector<float>> boosterPerWeek;      // per round, from a file
+    std::vector<std::vector<float>> boosterAgePercent;   // per round: 10 age-group shares
+    std::vector<int> order;                              // immunizationOrder: 10 values
+    std::vector<float> groupLevel;                       // vaccinationGroupLevel: 10 values
+};
+
 struct Scenario {
     Options opt;
+    VaccinationPlan vacc;
     LocationTypeIds lt;
     StateTable st;
     std::vector<ProgressionMatrixRaw> prog;

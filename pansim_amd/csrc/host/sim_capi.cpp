@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../../include/pansim_sim.h"
+#include "ensemble.h"
 #include "simulation.h"
 
 struct pansim_sim {
@@ -11,11 +12,17 @@ struct pansim_sim {
     std::string err;
 };
 
+struct pansim_ensemble {
+    std::unique_ptr<ps::Ensemble> ens;
+    std::string err;
+};
+
 namespace {
 std::string g_sim_create_error;
+std::string g_ensemble_create_error;
 
-template <class F>
-int guarded(pansim_sim* s, F&& f) {
+template <class H, class F>
+int guarded(H* s, F&& f) {
     try {
         f();
         return 0;
@@ -111,5 +118,85 @@ int pansim_sim_sizes(const pansim_sim* s, uint32_t* na, uint32_t* nl, uint32_t* 
 }
 
 pansim_ctx* pansim_sim_context(pansim_sim* s) { return s ? s->sim->ctx : nullptr; }
+
+// ---- ensembles
+int pansim_ensemble_create(pansim_ensemble** out, uint32_t n, const int* argc, const char* const* const* argv) {
+    if (!out || n == 0 || !argc || !argv) return PANSIM_E_ARG;
+    *out = nullptr;
+    std::vector<ps::Options> opts;
+    for (uint32_t r = 0; r < n; ++r) {
+        try {
+            opts.push_back(ps::parse_options(argc[r], argv[r], false));
+        } catch (const ps::InitError& e) {
+            g_ensemble_create_error = "replica " + std::to_string(r) + ": " + e.what();
+            return e.code;
+        }
+    }
+    auto* e = new pansim_ensemble();
+    e->ens.reset(new ps::Ensemble(opts));
+    if (!e->ens->ok()) {
+        g_ensemble_create_error = e->ens->initError;
+        int code = e->ens->initErrorCode < 0 ? e->ens->initErrorCode : PANSIM_E_DATA;
+        delete e;
+        return code;
+    }
+    *out = e;
+    return 0;
+}
+
+const char* pansim_ensemble_create_error(void) { return g_ensemble_create_error.c_str(); }
+const char* pansim_ensemble_error(const pansim_ensemble* e) { return e ? e->err.c_str() : g_ensemble_create_error.c_str(); }
+void pansim_ensemble_destroy(pansim_ensemble* e) { delete e; }
+
+int pansim_ensemble_steps(pansim_ensemble* e, uint32_t n) {
+    if (!e) return PANSIM_E_ARG;
+    return guarded(e, [&] {
+        for (uint32_t i = 0; i < n; ++i) e->ens->step();
+    });
+}
+
+int pansim_ensemble_run(pansim_ensemble* e) {
+    if (!e) return PANSIM_E_ARG;
+    return guarded(e, [&] {
+        while (e->ens->timestamp() < e->ens->totalSteps()) e->ens->step();
+    });
+}
+
+int pansim_ensemble_sync(pansim_ensemble* e) {
+    if (!e) return PANSIM_E_ARG;
+    return pansim_batch_sync(e->ens->batch);
+}
+
+uint32_t pansim_ensemble_size(const pansim_ensemble* e) { return e ? (uint32_t)e->ens->sims.size() : 0u; }
+uint32_t pansim_ensemble_timestamp(const pansim_ensemble* e) { return e ? e->ens->timestamp() : 0u; }
+uint32_t pansim_ensemble_steps_per_day(const pansim_ensemble* e) {
+    return e ? e->ens->sims.front()->getSimTime().getStepsPerDay() : 0u;
+}
+
+pansim_ctx* pansim_ensemble_context(pansim_ensemble* e, uint32_t r) {
+    return (e && r < e->ens->sims.size()) ? e->ens->sims[r]->ctx : nullptr;
+}
+
+int pansim_ensemble_num_days(const pansim_ensemble* e, uint32_t r) {
+    return (e && r < e->ens->sims.size()) ? (int)e->ens->sims[r]->dayStats.size() : -1;
+}
+
+int pansim_ensemble_day_stats(const pansim_ensemble* e, uint32_t r, int d, uint32_t* out, size_t cap, size_t* len) {
+    if (!e || r >= e->ens->sims.size()) return PANSIM_E_ARG;
+    const auto& ds = e->ens->sims[r]->dayStats;
+    if (d < 0 || d >= (int)ds.size()) return PANSIM_E_ARG;
+    if (len) *len = ds[d].size();
+    for (size_t i = 0; out && i < cap && i < ds[d].size(); ++i) out[i] = ds[d][i];
+    return 0;
+}
+
+int pansim_ensemble_sizes(const pansim_ensemble* e, uint32_t r, uint32_t* na, uint32_t* nl, uint32_t* nv) {
+    if (!e || r >= e->ens->sims.size()) return PANSIM_E_ARG;
+    const auto& s = *e->ens->sims[r];
+    if (na) *na = s.sc.agents.n;
+    if (nl) *nl = s.sc.locs.n;
+    if (nv) *nv = (uint32_t)s.sc.variantMultiplier.size();
+    return 0;
+}
 
 }  // extern "C"

@@ -183,6 +183,12 @@ public:
     void infectionsAtLocations(const Timehandler& simTime, unsigned timeStep, uint8_t variant) {
         infectionsAtLocationsMask(simTime, timeStep, 1u << variant);
     }
+    // the arguments infectionsAtLocationsMask passes for this day (k and the
+    // seasonality are constant within a day)
+    void dayInfectionParams(const Timehandler& simTime, float& kOut, float& seasonOut) const {
+        kOut = (float)k;
+        seasonOut = (float)seasonalityMultiplier(simTime);
+    }
 };
 
 // ---------------------------------------------------------------- testing
@@ -525,6 +531,11 @@ public:
             int rc = pansim_create(&ctx, &model.dims, &model.tables, &model.locations, &model.agents, o.seed, replica,
                                    timeStep, o.quarantinePolicy, o.quarantineLength);
             if (rc != 0) throw InitError(rc, std::string("pansim_create: ") + pansim_create_error());
+            {  // Immunization::initCategories (immunization.h:229-420)
+                int32_t order[10];
+                for (int g = 0; g < 10; ++g) order[g] = sc.vacc.order[g];
+                check(ctx, pansim_vaccination_init(ctx, order, sc.vacc.groupLevel.data()), "initCategories");
+            }
             Movement::init(sc.lt, sc.locs.cemetery);
             Testing::init(sc.lt);
             statesHeader = sc.statesHeader;
@@ -557,9 +568,8 @@ public:
     // simulation.h:425-471
     void updateAgents(Timehandler& t) { check(ctx, pansim_progression_day(ctx, t.getTimestamp(), timeStep), "updateAgents"); }
 
-    // Immunization::update (immunization.h:481-900): the susceptibility pass.
-    // The vaccination and booster rollout (from day immunizationStart /
-    // boosterStart) is not part of this build and raises instead.
+    // Immunization::update (immunization.h:481-900): the susceptibility pass,
+    // the booster rounds, then the first doses.
     void immunizationUpdate(Timehandler& t) {
         auto f = [](const std::string& s) { return split_float(s, ','); };
         auto pi = f(sc.opt.protectionInfection), piw = f(sc.opt.protectionInfectionWaning);
@@ -583,12 +593,55 @@ public:
         for (size_t i = 0; i < vs.size() && i < 7; ++i) sim[i] = vs[i];
         for (size_t i = 0; i < vm.size() && i < 7; ++i) simMul[i] = vm[i];
         check(ctx, pansim_immunization_day(ctx, t.getTimestamp(), timeStep, prot, sim, simMul), "immunization");
-        unsigned simDay = t.getTimestamp() / t.getStepsPerDay();
-        bool active = simDay >= sc.opt.immunizationStart;
-        for (int b : split_int(sc.opt.boosterStart, ',')) active = active || (int)simDay >= b;
-        if (active)
-            throw InitError(ERR_UNSUPPORTED, "vaccination/booster rollout (day >= immunizationStart/boosterStart) is not implemented");
-        immunizedToday = 0;
+        vaccinationRollout(t);
+    }
+
+    // initAgeGroups (immunization.h:421-479): booster shares per age group
+    void initAgeGroups() {
+        ageGroupsReady_ = true;
+        uint32_t ages[10];
+        check(ctx, pansim_age_group_counts(ctx, ages), "initAgeGroups");
+        const auto& v = sc.vacc;
+        ageGroupFrac_.assign(v.boosterStart.size(), std::array<float, 10>{});
+        for (size_t r = 0; r < v.boosterStart.size(); ++r) {
+            unsigned size[10], sum = 0;
+            for (int g = 0; g < 10; ++g) {
+                size[g] = (unsigned)((float)ages[g] * v.boosterAgePercent[r][g]);
+                sum += size[g];
+            }
+            for (int g = 0; g < 10; ++g) ageGroupFrac_[r][g] = (float)size[g] / (float)sum;
+        }
+    }
+
+    // numberOfBoostersToday / numberOfVaccinesToday (immunization.h:62-89) and the rollout
+    void vaccinationRollout(Timehandler& t) {
+        if (!ageGroupsReady_) initAgeGroups();
+        const auto& v = sc.vacc;
+        const unsigned ts = t.getTimestamp();
+        const unsigned day = ts / t.getStepsPerDay();
+        for (size_t r = 0; r < v.boosterStart.size(); ++r) {
+            unsigned available = 0;
+            if ((int)day >= v.boosterStart[r]) {
+                if (v.boosterPerWeek[r].empty()) available = v.dailyBoosters;
+                else {
+                    unsigned week = (day - (unsigned)v.boosterStart[r]) / 7;
+                    const auto& w = v.boosterPerWeek[r];
+                    available = (unsigned)(w[week >= w.size() ? w.size() - 1 : week] / 7.0);
+                }
+            }
+            check(ctx, pansim_booster_round(ctx, ts, timeStep, (uint32_t)r, available, ageGroupFrac_[r].data()), "boosters");
+        }
+        unsigned available = 0;
+        if (day >= v.startDay) {
+            if (v.vaccPerWeek.empty()) available = v.dailyDoses;
+            else {
+                unsigned week = (day - v.startDay) / 7;
+                available = (unsigned)(v.vaccPerWeek[week >= v.vaccPerWeek.size() ? v.vaccPerWeek.size() - 1 : week] / 7.0);
+            }
+        }
+        uint32_t today = 0;
+        check(ctx, pansim_vaccinate(ctx, ts, timeStep, available, &currentCategory_, &today), "vaccination");
+        immunizedToday = today;
     }
 
     // simulation.h:169-204 host part + kernel
@@ -640,6 +693,21 @@ public:
         Closure::midnight(t, timeStep, stats);
         Movement::planLocations(t, timeStep);
         variantCounts = countVariantCases();
+    }
+
+    // The parts of one step that a replica batch (pansim_amd/csrc/host/ensemble.h)
+    // runs per replica around its batched movement and infection launches.
+    void midnightIfDue() {
+        if (simTime_.isMidnight()) midnight();
+    }
+    pansim_step_params batchStepParams() const { return Movement::stepParams(simTime_, timeStep); }
+    void batchInfectionParams(uint32_t& mask, float& kOut, float& seasonOut) const {
+        mask = activeMask();
+        Infection::dayInfectionParams(simTime_, kOut, seasonOut);
+    }
+    void finishBatchedStep() {
+        Closure::step(simTime_, timeStep);
+        ++simTime_;
     }
 
     // One iteration of runSimulation's loop (simulation.h:904-923).
@@ -790,6 +858,9 @@ public:
 
 private:
     Timehandler simTime_;
+    uint32_t currentCategory_ = 0;
+    bool ageGroupsReady_ = false;
+    std::vector<std::array<float, 10>> ageGroupFrac_;
     std::vector<Event> schedulesHost_;
     std::vector<uint32_t> offsetsHost_;
 };

@@ -68,3 +68,11 @@ def test_parity_mutation_and_rules(tmp_path):
     p.write_text(json.dumps(rules))
     args = pansim_amd.config1_args(2500, 500, 1, 21, extra=["-k", "0.03", "--closures", str(p), "--enableClosures", "1"])
     run_parity(args, [144, 300, 600, 1008])
+
+
+def test_parity_vaccination_and_boosters():
+    # first doses from day 2, boosters from day 125 (immunization.h:62-900):
+    # every field, including the category rounds and immunization stats
+    args = pansim_amd.config1_args(1500, 300, 19, 4, extra=[
+        "-k", "0.005", "--immunizationStart", "2", "--immunizationsPerWeek", "700", "--boosterStart", "125"])
+    run_parity(args, [289, 1008, 126 * 144 + 1, 19 * 1008])

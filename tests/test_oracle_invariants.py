@@ -153,3 +153,38 @@ def test_oracle_ignores_unknown_flags(flag):
     finally:
         a.close()
         b.close()
+
+
+def test_oracle_vaccination_rollout():
+    # first doses from day 2 (100 per day), boosters from day 125 (default
+    # weekly booster table); immunization.h:62-89, 696-900
+    args = pansim_amd.config1_args(1500, 300, 19, 4, extra=[
+        "-k", "0.005", "--immunizationStart", "2", "--immunizationsPerWeek", "700", "--boosterStart", "125"])
+    o = OracleSim(args)
+    try:
+        o.steps(2 * 144)
+        assert (o.field("immunizationCount") == 0).all()
+        o.steps(1)  # midnight of day 2: the first doses
+        it = o.field("immunizationTimestamp")
+        ic = o.field("immunizationCount")
+        rnd = o.field("vaccinationRound")
+        first = ic == 1
+        assert 0 < first.sum() <= 200
+        assert (it[first] == 288).all()
+        # only willing agents of a category with a round get a dose
+        assert ((rnd[first] >= 1) & (rnd[first] <= 10)).all()
+        wb, _ = o.state_table()
+        ns = len(wb)
+        st = o.day_stats(o.num_days - 1)
+        nv = (len(st) - ns - 16) // 3
+        vac = st[ns + 7 + max(1, nv - 1) + 1]
+        assert vac == int((it == 288).sum())
+        # boosters: only on or after day 125, only for agents with a first dose
+        o.steps(19 * 1008 - o.timestamp)
+        it = o.field("immunizationTimestamp")
+        ic = o.field("immunizationCount")
+        boosted = ic >= 2
+        assert boosted.sum() > 0
+        assert (it[boosted] >= 125 * 144).all()
+    finally:
+        o.close()
