@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-agents", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=1008)
+    ap.add_argument("--ensemble-replicas", type=int, default=64,
+                    help="replicas of the batched-ensemble measurement (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=288)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -90,6 +93,68 @@ def cpu_baseline(a):
         "kind": "port",
         "sample": f"oracle (sequential C++ re-statement of the reference pipeline), {o.n_agents} synthetic agents, "
                   f"{a.cpu_steps} steps incl. the day-0 midnight block, {dt:.1f} s",
+    }
+
+
+def ensemble_args(n_rep, weeks):
+    """Replica command lines of the batched-ensemble workload: sizes
+    log-uniform in [1e5, 2e6], seed_i = i, k_i uniform in [2e-4, 6e-4],
+    closure rules on for even replicas, quarantine policy i % 4."""
+    import numpy as np
+    import pansim_amd
+
+    rng = np.random.default_rng(2024)
+    sizes = np.exp(rng.uniform(np.log(1e5), np.log(2e6), n_rep)).astype(int)
+    ks = rng.uniform(2e-4, 6e-4, n_rep)
+    out = []
+    for i in range(n_rep):
+        extra = ["-k", f"{ks[i]:.6g}", "--quarantinePolicy", str(i % 4)]
+        if i % 2:
+            extra += ["--enableClosures", "0", "--closures", "inputConfigFiles/noRules.json"]
+        out.append(pansim_amd.synthetic_args(int(sizes[i]), weeks, i, i + 1, extra=extra))
+    return out, int(sizes.sum())
+
+
+def ensemble_measure(a):
+    """The same replicas stepped as one batch (one launch per kernel per step
+    for all of them) and one after the other with the plain path; day 0 is
+    warmup, then --ensemble-steps steps are timed in each mode."""
+    import pansim_amd
+
+    reps, total = ensemble_args(a.ensemble_replicas, a.weeks)
+    K = a.ensemble_steps
+    t0 = time.perf_counter()
+    ens = pansim_amd.Ensemble(reps)
+    log(f"ensemble: {a.ensemble_replicas} replicas, {total} agents built in {time.perf_counter() - t0:.1f}s")
+    ens.steps(144)
+    ens.sync()
+    t1 = time.perf_counter()
+    ens.steps(K)
+    ens.sync()
+    batched = time.perf_counter() - t1
+    ens.close()
+    log(f"ensemble: batched {K} steps in {batched:.3f}s")
+    looped = 0.0
+    for args in reps:
+        sim = pansim_amd.Simulation(args)
+        sim.steps(144)
+        sim.sync()
+        t1 = time.perf_counter()
+        sim.steps(K)
+        sim.sync()
+        looped += time.perf_counter() - t1
+        sim.close()
+    log(f"ensemble: looped {K} steps in {looped:.3f}s")
+    return {
+        "workload": f"{a.ensemble_replicas} synthetic replicas, sizes log-uniform in [1e5, 2e6] (sum {total} agents), "
+                    "seed_i = i, k_i ~ U[2e-4, 6e-4], closures on/off, quarantine policy i % 4, dt = 10 min",
+        "replicas": a.ensemble_replicas,
+        "agents": total,
+        "steps": K,
+        "warmup": 144,
+        "batched_agent_steps_per_s": total * K / batched,
+        "looped_agent_steps_per_s": total * K / looped,
+        "speedup": looped / batched,
     }
 
 
@@ -145,6 +210,10 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_step")
     last = sim.day_stats(sim.num_days - 1)
+    sim.close()
+    ensemble = None
+    if a.ensemble_replicas > 0:
+        ensemble = ensemble_measure(a)
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -186,6 +255,7 @@ def main():
             "algorithmic_bytes_per_step": step_bytes_total / K,
         },
         "final_day_stats": last[:12],
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
