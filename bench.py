@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--ensemble-replicas", type=int, default=64,
                     help="replicas of the batched-ensemble measurement (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=288)
+    ap.add_argument("--gradient-replicas", type=int, default=32,
+                    help="replicas of the gradient measurement (0 skips it)")
+    ap.add_argument("--gradient-agents", type=int, default=1_000_000)
+    ap.add_argument("--gradient-weeks", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -158,6 +162,45 @@ def ensemble_measure(a):
     }
 
 
+def gradient_measure(a):
+    """One evaluation of the expected cumulative infections and their
+    gradient with respect to k and the location-type multipliers: the
+    replicas (seed_i = i) run as one batch for the whole horizon with hazard
+    accumulation on; the populations are built before the timed region."""
+    import numpy as np
+    import pansim_amd
+
+    reps = [pansim_amd.synthetic_args(a.gradient_agents, a.gradient_weeks, i, i + 1) for i in range(a.gradient_replicas)]
+    t0 = time.perf_counter()
+    ens = pansim_amd.Ensemble(reps)
+    log(f"gradient: {a.gradient_replicas} x {a.gradient_agents} agents built in {time.perf_counter() - t0:.1f}s")
+    views = [ens.replica(r) for r in range(a.gradient_replicas)]
+    m = np.ones(256, dtype=np.float32)
+    steps = a.gradient_weeks * 7 * 144
+    ens.sync()
+    t1 = time.perf_counter()
+    for v in views:
+        v.set_location_type_multipliers(m)
+        v.hazard_enable(True)
+    ens.steps(steps)
+    ens.sync()
+    J = dk = 0.0
+    for v in views:
+        j, g, _ = v.hazard()
+        J, dk = J + j, dk + g
+    dt = time.perf_counter() - t1
+    ens.close()
+    log(f"gradient: one evaluation ({steps} steps) in {dt:.2f}s")
+    return {
+        "workload": f"{a.gradient_replicas} synthetic replicas x {a.gradient_agents} agents, {a.gradient_weeks} weeks, "
+                    "gradient of the expected cumulative infections w.r.t. k and the location-type multipliers",
+        "gradient_evals_per_s": 1.0 / dt,
+        "agent_steps_per_s": a.gradient_replicas * a.gradient_agents * steps / dt,
+        "mean_expected_infections": J / a.gradient_replicas,
+        "mean_dJ_dk": dk / a.gradient_replicas,
+    }
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -214,6 +257,9 @@ def main():
     ensemble = None
     if a.ensemble_replicas > 0:
         ensemble = ensemble_measure(a)
+    gradient = None
+    if a.gradient_replicas > 0:
+        gradient = gradient_measure(a)
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -256,6 +302,7 @@ def main():
         },
         "final_day_stats": last[:12],
         "ensemble": ensemble,
+        "gradient": gradient,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -298,6 +298,20 @@ int pansim_device_expf(const float* in, float* out, size_t n);
 int pansim_device_log(const double* in, double* out, size_t n);
 int pansim_device_philox(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4, size_t n);
 
+/* ---- Hazard gradient -----------------------------------------------------
+ * Calibration support, with no counterpart in the reference: per location
+ * type multipliers m_t of the location infectiousness (type id < 256; all 1
+ * reproduce the reference bit for bit), and the expected number of
+ * infections J = sum over infection steps, variants and susceptible agents
+ * at a location with infectious visitors of s * (1 - exp(-x)), x = k *
+ * density * infectiousness * m_type * seasonality * dt, with its gradient
+ * with respect to k and every m_t along the simulated trajectory (which is
+ * held fixed), accumulated in FP64 while enabled.  out[0] = J, out[1] =
+ * dJ/dk, out[2 + t] = dJ/dm_t; n >= 258. */
+int pansim_set_location_type_multipliers(pansim_ctx* ctx, const float* m, uint32_t n);
+int pansim_hazard_enable(pansim_ctx* ctx, uint32_t on);
+int pansim_hazard_read(pansim_ctx* ctx, double* out, uint32_t n, uint32_t reset);
+
 /* ---- Replica batches --------------------------------------------------
  * Many contexts (replicas of uneven size, own seed, k, policies) on one
  * device stepped together: every step is one movement, one quarantine-write

@@ -248,6 +248,17 @@ public:
     std::vector<unsigned> variantCounts;
     unsigned healthcareWorkerCount = 0;
     unsigned immunizedToday = 0;
+    // calibration interface: location-type multipliers, expected-infection
+    // hazard and its gradient (FP64), hazard sums at perturbed parameters
+    std::vector<float> typeMul = std::vector<float>(256, 1.0f);
+    bool hazardOn = false;
+    std::vector<double> hazard = std::vector<double>(258, 0.0);
+    struct Probe {
+        double dk;
+        std::vector<double> dm;
+    };
+    std::vector<Probe> probes;
+    std::vector<double> probeJ;
     // vaccination rollout (immunization.h:24-61)
     std::vector<uint8_t> immRound;  // category round of every agent, 255 = not willing
     unsigned currentCategory = 0;
@@ -929,7 +940,31 @@ void Sim::infectionsAtLocations(unsigned variant) {
             continue;
         }
         float density = num / (float)locCount[l];
-        ratio[l] = 1.0f - exp_f32(-tmpK * density * locInf[l] * season * (float)dt);
+        // location-type multiplier of the calibration interface (1 = reference)
+        ratio[l] = 1.0f - exp_f32(-tmpK * density * (locInf[l] * typeMul[locType[l] & 255u]) * season * (float)dt);
+    }
+    if (hazardOn) {
+        // expected infections and their gradient along the trajectory, all in
+        // FP64 (x recomputed in double), plus the hazard sums of the probes
+        for (unsigned i = 0; i < N; ++i) {
+            const double s = pp[i].susceptible[variant];
+            const unsigned l = location[i];
+            if (!(s > 0.0) || sums[l] == 0) continue;
+            const double density = (double)(float)((double)sums[l] * (1.0 / 4294967296.0)) / (double)locCount[l];
+            const unsigned t = locType[l] & 255u;
+            const double base = density * (double)locInf[l] * (double)season * (double)dt;
+            const double m = typeMul[t];
+            const double x = (double)tmpK * base * m;
+            const double ex = std::exp(-x);
+            hazard[0] += s * (1.0 - ex);
+            hazard[1] += s * ex * base * m;
+            hazard[2 + t] += s * ex * (double)tmpK * base;
+            for (size_t q = 0; q < probes.size(); ++q) {
+                const Probe& pr = probes[q];
+                const double xp = ((double)tmpK + pr.dk) * base * (m + pr.dm[t]);
+                probeJ[q] += s * (1.0 - std::exp(-xp));
+            }
+        }
     }
     for (unsigned i = 0; i < N; ++i) {
         PPState& p = pp[i];
@@ -2116,6 +2151,30 @@ void oracle_state_table(void* h, unsigned char* wb, float* infectious) {
 void oracle_agent_infectious(void* h, float* out) {
     auto* s = (oracle::Sim*)h;
     for (unsigned i = 0; i < s->N; ++i) out[i] = s->pp[i].infectious;
+}
+
+// Calibration interface (hazard of expected infections and its gradient).
+void oracle_set_type_multipliers(void* h, const float* m, unsigned n) {
+    auto* s = (oracle::Sim*)h;
+    for (unsigned t = 0; t < 256; ++t) s->typeMul[t] = t < n ? m[t] : 1.0f;
+}
+void oracle_hazard_enable(void* h, int on) {
+    auto* s = (oracle::Sim*)h;
+    s->hazardOn = on != 0;
+    std::fill(s->hazard.begin(), s->hazard.end(), 0.0);
+}
+// a hazard probe at k + dk, m + dm[256] (perturbations, FP64)
+int oracle_hazard_probe(void* h, double dk, const double* dm) {
+    auto* s = (oracle::Sim*)h;
+    oracle::Sim::Probe p{dk, std::vector<double>(dm, dm + 256)};
+    s->probes.push_back(p);
+    s->probeJ.push_back(0.0);
+    return (int)s->probes.size() - 1;
+}
+void oracle_hazard_read(void* h, double* out, double* probeJ) {
+    auto* s = (oracle::Sim*)h;
+    for (int j = 0; j < 258; ++j) out[j] = s->hazard[j];
+    for (size_t q = 0; probeJ && q < s->probeJ.size(); ++q) probeJ[q] = s->probeJ[q];
 }
 
 // Known-answer helpers for the tests.

@@ -49,6 +49,11 @@ def lib():
     L.oracle_num_states.restype = ctypes.c_uint32
     L.oracle_state_table.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)]
     L.oracle_agent_infectious.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.oracle_set_type_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
+    L.oracle_hazard_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_hazard_probe.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+    L.oracle_hazard_read.argtypes = [ctypes.c_void_p, dp, dp]
     L.oracle_philox.argtypes = [u32p, ctypes.c_uint32, ctypes.c_uint32, u32p]
     L.oracle_expf.argtypes = [ctypes.c_float]
     L.oracle_expf.restype = ctypes.c_float
@@ -133,6 +138,28 @@ class OracleSim:
 
     def seasonality(self, ts: int) -> float:
         return lib().oracle_seasonality(self._h, ts)
+
+    def set_type_multipliers(self, m):
+        a = np.ascontiguousarray(m, dtype=np.float32)
+        lib().oracle_set_type_multipliers(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(a))
+
+    def hazard_enable(self, on=True):
+        self._probes = 0
+        lib().oracle_hazard_enable(self._h, 1 if on else 0)
+
+    def hazard_probe(self, dk=0.0, dm=None):
+        """Adds a probe: the hazard sum at (k + dk, m + dm) along the same trajectory."""
+        d = np.zeros(256) if dm is None else np.ascontiguousarray(dm, dtype=np.float64)
+        self._probes = getattr(self, "_probes", 0) + 1
+        return lib().oracle_hazard_probe(self._h, dk, d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def hazard(self):
+        """(J, dJ/dk, dJ/dm[256], probe sums)."""
+        out = np.zeros(258)
+        pj = np.zeros(max(1, getattr(self, "_probes", 0)))
+        lib().oracle_hazard_read(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 pj.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out[0], out[1], out[2:], pj[: getattr(self, "_probes", 0)]
 
     def csr(self):
         """(offsets[N+1], possible locations, their location types) of every agent."""

@@ -108,6 +108,9 @@ def lib() -> ctypes.CDLL:
     L.pansim_ensemble_day_stats.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, c_u32p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]
     L.pansim_ensemble_sizes.argtypes = [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_u32p, c_u32p]
+    L.pansim_set_location_type_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
+    L.pansim_hazard_enable.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_hazard_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32]
     _lib = L
     return L
 
@@ -234,6 +237,22 @@ class Simulation:
         if rc != 0:
             raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
 
+    def set_location_type_multipliers(self, m):
+        """Infectiousness multiplier of every location type id (1 = reference)."""
+        a = np.ascontiguousarray(m, dtype=np.float32)
+        self._check_ctx(lib().pansim_set_location_type_multipliers(
+            self.ctx, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(a)))
+
+    def hazard_enable(self, on: bool = True):
+        self._check_ctx(lib().pansim_hazard_enable(self.ctx, 1 if on else 0))
+
+    def hazard(self, reset: bool = False):
+        """(J, dJ/dk, dJ/dm[256]): expected infections and their gradient so far."""
+        out = np.zeros(258)
+        self._check_ctx(lib().pansim_hazard_read(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 258,
+                                                 1 if reset else 0))
+        return out[0], out[1], out[2:]
+
     def location_sums(self, variant: int) -> np.ndarray:
         out = np.zeros(self.n_locations, dtype=np.int64)
         rc = lib().pansim_download_location_sums(self.ctx, variant, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
@@ -329,6 +348,10 @@ class EnsembleReplica:
 
     field = Simulation.field
     location_sums = Simulation.location_sums
+    set_location_type_multipliers = Simulation.set_location_type_multipliers
+    hazard_enable = Simulation.hazard_enable
+    hazard = Simulation.hazard
+    _check_ctx = Simulation._check_ctx
 
 
 class SimulatorInterface:

@@ -43,6 +43,9 @@ struct pansim_ctx {
     uint32_t evOffCap = 0, evCap = 0;
     uint32_t nEvents = 0;
     int2* evWinDev = nullptr;
+    float* typeMul = nullptr;  // [HAZARD_TYPES]
+    double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
+    bool hazardOn = false;
     uint32_t dt = 10;
     uint32_t wakeDay = 0;  // first step of the day the stored wake steps count from
 };
@@ -279,6 +282,13 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     }
     TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));
     TRY(dalloc(c, &L.pendKey, NL));
+    {
+        std::vector<float> ones(HAZARD_TYPES, 1.0f);
+        TRY(upload(c, &c->typeMul, ones.data(), HAZARD_TYPES));
+        TRY(dalloc(c, &c->hazard, HAZARD_SLOTS));
+        HIP_OR_FAIL(c, hipMemsetAsync(c->hazard, 0, HAZARD_SLOTS * sizeof(double), c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    }
     TRY(upload(c, (uint32_t**)&L.schools, l->schools, d->n_schools));
     TRY(upload(c, (uint32_t**)&L.classrooms, l->classrooms, d->n_classrooms));
     TRY(upload(c, (uint32_t**)&L.classOff, l->classroom_offsets, d->n_schools + 1));
@@ -487,6 +497,8 @@ static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, 
     a.k = k;
     a.season = season;
     a.dt = (float)dt;
+    a.typeMul = c->typeMul;
+    a.hazard = c->hazardOn ? c->hazard : nullptr;
     return a;
 }
 
@@ -1012,6 +1024,32 @@ int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t
     hipFree(din);
     hipFree(dout);
     return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+// ------------------------------------------------------------ hazard gradient
+int pansim_set_location_type_multipliers(pansim_ctx* c, const float* m, uint32_t n) {
+    if (!c || (n && !m) || n > HAZARD_TYPES) return PANSIM_E_ARG;
+    std::vector<float> v(HAZARD_TYPES, 1.0f);
+    for (uint32_t t = 0; t < n; ++t) v[t] = m[t];
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    HIP_OR_FAIL(c, hipMemcpy(c->typeMul, v.data(), HAZARD_TYPES * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pansim_hazard_enable(pansim_ctx* c, uint32_t on) {
+    if (!c) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    HIP_OR_FAIL(c, hipMemset(c->hazard, 0, HAZARD_SLOTS * sizeof(double)));
+    c->hazardOn = on != 0;
+    return 0;
+}
+
+int pansim_hazard_read(pansim_ctx* c, double* out, uint32_t n, uint32_t reset) {
+    if (!c || !out || n < HAZARD_SLOTS) return PANSIM_E_ARG;
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    HIP_OR_FAIL(c, hipMemcpy(out, c->hazard, HAZARD_SLOTS * sizeof(double), hipMemcpyDeviceToHost));
+    if (reset) HIP_OR_FAIL(c, hipMemset(c->hazard, 0, HAZARD_SLOTS * sizeof(double)));
+    return 0;
 }
 
 // ------------------------------------------------------------ replica batches

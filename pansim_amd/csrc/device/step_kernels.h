@@ -661,6 +661,9 @@ __global__ void __launch_bounds__(256) k_resolve_pending(Locs L, StepScratch S, 
     resolve_pending(L, S, par, blockIdx.x, gridDim.x);
 }
 
+constexpr uint32_t HAZARD_TYPES = 256;  // location-type multipliers, by type id
+constexpr uint32_t HAZARD_SLOTS = 2 + HAZARD_TYPES;  // J, dJ/dk, dJ/dm[type]
+
 struct InfectArgs {
     Agents A;
     Locs L;
@@ -673,6 +676,8 @@ struct InfectArgs {
     float k;
     float season;
     float dt;
+    const float* typeMul;  // [HAZARD_TYPES] infectiousness multiplier of each location type (1 = reference)
+    double* hazard;        // [HAZARD_SLOTS] accumulated expected infections and gradient, or null
 };
 
 // Infection of all active variants.  LocationsList::infectAgents
@@ -684,12 +689,30 @@ constexpr uint32_t INFECT_BLOCK = 256;
 constexpr uint32_t INFECT_ITEMS = 8;
 constexpr uint32_t INFECT_TILE = INFECT_BLOCK * INFECT_ITEMS;
 
+// The expected-infection hazard of one susceptible agent at a location with
+// infectious visitors, s * (1 - exp(-x)) with x = k * density * inf * m_type *
+// season * dt, and its derivatives with respect to k and m_type, with the
+// trajectory held fixed (FP64, accumulated into LDS slots of the block).
+__device__ __forceinline__ void hazard_terms(double* sh, float k, float density, float inf, float m, uint32_t type,
+                                             float season, float dt, float x, float s) {
+    const double ex = exp(-(double)x);
+    const double w = (double)s * ex;
+    atomicAdd(&sh[0], (double)s * (1.0 - ex));
+    atomicAdd(&sh[1], w * ((double)density * (double)inf * (double)m * (double)season * (double)dt));
+    atomicAdd(&sh[2 + type], w * ((double)k * (double)density * (double)inf * (double)season * (double)dt));
+}
+
 __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
-                                             uint32_t count, float inf, float s) {
+                                             uint32_t count, float inf, float s, double* sh) {
     const uint32_t n = a.A.n;
     float num = (float)((double)e * (1.0 / 4294967296.0));
     float density = num / (float)count;
-    float ratio = 1.0f - ps_expf(-a.k * density * inf * a.season * a.dt);
+    const uint32_t type = a.L.rec[loc].type;
+    const float m = a.typeMul[type];
+    const float infm = inf * m;
+    const float x = a.k * density * infm * a.season * a.dt;
+    if (sh) hazard_terms(sh, a.k, density, inf, m, type, a.season, a.dt, x, s);
+    float ratio = 1.0f - ps_expf(-x);
     if (ratio == 0.0f) return;
     Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
     if (rs.unit() < (double)(ratio * s)) {
@@ -711,6 +734,13 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
 
 __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) {
     if (a.mask == 0u) return;
+    __shared__ double shHazard[HAZARD_SLOTS];
+    double* sh = nullptr;
+    if (a.hazard) {  // uniform over the grid
+        for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
+        __syncthreads();
+        sh = shHazard;
+    }
     const uint32_t n = a.A.n;
     const uint32_t base = tile * INFECT_TILE + threadIdx.x;
     uint32_t loc[INFECT_ITEMS];
@@ -743,8 +773,13 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
             const uint32_t i = base + k * INFECT_BLOCK;
             const uint32_t l = a.A.location[i];
             const LocDyn d = a.L.dyn[l];
-            infect_agent(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, a.A.susc[(size_t)v * n + i]);
+            infect_agent(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, a.A.susc[(size_t)v * n + i], sh);
         }
+    }
+    if (sh) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x)
+            if (shHazard[j] != 0.0) atomicAdd(&a.hazard[j], shHazard[j]);
     }
 }
 
