@@ -42,7 +42,7 @@ __device__ __forceinline__ float scaling_symptoms(const Agents& A, uint32_t i, u
 
 __device__ __forceinline__ void update_meta(const Agents& A, const Tables& T, uint32_t i, uint8_t st) {
     float sf = T.stSusc[st] ? 1.0f : 0.0f;
-    for (uint32_t v = 0; v < T.nVar; ++v) A.susc[(size_t)v * A.n + i] = sf;
+    for (uint32_t v = 0; v < T.nVar; ++v) A.susc[(size_t)v * A.suscStride + i] = sf;
 }
 
 // ------------------------------------------------------------ init
@@ -95,17 +95,40 @@ __global__ void k_zero_locations(Locs L, uint32_t nVar) {
         L.dyn[l].count = 0u;
         L.dyn[l].sum0 = 0;
         for (uint32_t v = 1; v < nVar; ++v) L.sumX[(size_t)(v - 1) * L.n + l] = 0;
+        if ((l & 31u) == 0) L.visited[l >> 5] = 0u;
     }
+}
+
+// Runs of equal values among the consecutive lanes of a wave: `head` is set
+// on the first lane of each run, the return value is the run's length there.
+// Valid lanes form a prefix of the wave.
+__device__ __forceinline__ uint32_t lane_run(uint32_t v, bool valid, bool& head) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(v, 1, 64);
+    head = valid && (lane == 0 || prev != v);
+    const unsigned long long heads = __ballot(head);
+    const uint32_t nValid = (uint32_t)__popcll(__ballot(valid));
+    const unsigned long long after = heads & ~((2ull << lane) - 1ull);
+    uint32_t next = after ? (uint32_t)(__ffsll((long long)after) - 1) : 64u;
+    if (next > nValid) next = nValid;
+    return next - lane;
 }
 
 __global__ void k_scatter_locations(Agents A, Locs L, Tables T) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
-    uint32_t loc = A.location[i];
-    atomicAdd(&L.dyn[loc].count, 1u);
+    const bool valid = i < A.n;
+    const uint32_t loc = valid ? A.location[i] : NOLOC;
+    // members of a household are consecutive agents: one occupancy atomic per run
+    bool head;
+    const uint32_t run = lane_run(loc, valid, head);
+    if (!valid) return;
+    if (head) atomicAdd(&L.dyn[loc].count, run);
     const AgentRec r = A.rec[i];
     float x = T.stInf[r.state] * T.varMul[r.variant];
-    if (x != 0.0f) atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)fixed32(x));
+    if (x != 0.0f) {
+        atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)fixed32(x));
+        mark_visited(L, loc);
+    }
 }
 
 __global__ void k_count_variants(Agents A, Tables T, unsigned long long* out) {
@@ -146,15 +169,16 @@ __global__ void k_flag_locations(TestArgs a) {
         uint32_t q = a.L.rec[l].quarUntil;
         return q == 0 || (q != 0 && (q - ql >= a.ts - day || q < a.ts));
     };
-    uint32_t home = find_loc(a.A, a.L, lt, i, lt.home, 0, nullptr);
+    const CsrReg csr = load_csr(a.A, a.A.rec[i]);
+    uint32_t home = csr.home;
     if (home != NOLOC) a.L.rec[home].flag = 1;
-    uint32_t work = find_loc(a.A, a.L, lt, i, lt.work, 0, nullptr);
+    uint32_t work = find_loc(a.A, a.L, lt, csr, lt.work, 0, nullptr);
     if (work != NOLOC && ok(work)) a.L.rec[work].flag = 1;
-    uint32_t school = find_loc(a.A, a.L, lt, i, lt.school, 0, nullptr);
+    uint32_t school = find_loc(a.A, a.L, lt, csr, lt.school, 0, nullptr);
     if (school != NOLOC && ok(school)) {
         a.L.rec[school].flag = 1;
         Stream rs(a.seed, i, a.ts, PH_TEST_FLAG, a.replica);
-        uint32_t cls = find_loc(a.A, a.L, lt, i, lt.classroom, 0, &rs);
+        uint32_t cls = find_loc(a.A, a.L, lt, csr, lt.classroom, 0, &rs);
         if (cls != NOLOC && ok(cls)) a.L.rec[cls].flag = 1;
     }
 }
@@ -175,15 +199,16 @@ __global__ void k_do_testing(TestArgs a) {
         uint8_t st = A.rec[i].state;
         if (!skip && !(a.T.stWB[st] == 4 || (fl & F_DIAGNOSED))) {
             Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
-            uint32_t home = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
+            const CsrReg csr = load_csr(A, A.rec[i]);
+            uint32_t home = csr.home;
             bool homeFlag = home != NOLOC ? a.L.rec[home].flag != 0 : false;
-            uint32_t work = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
+            uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
             bool workFlag = work != NOLOC ? a.L.rec[work].flag != 0 : false;
-            uint32_t school = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
+            uint32_t school = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
             bool schoolFlag = false, classFlag = false;
             if (school != NOLOC) {
                 schoolFlag = a.L.rec[school].flag != 0;
-                uint32_t cls = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
+                uint32_t cls = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
                 if (cls != NOLOC) classFlag = true;
             }
             double prob = a.pRandom;
@@ -396,7 +421,7 @@ __global__ void k_immunization(ImmArgs a) {
             setSlot(v, fmin_ref(1.0f, (1.0f - vsHosp) / (1.0f - vsSympt)));
         }
     }
-    for (uint32_t v = 0; v < nVar; ++v) A.susc[(size_t)v * A.n + i] = fmin_ref(1.0f, susc[v]);
+    for (uint32_t v = 0; v < nVar; ++v) A.susc[(size_t)v * A.suscStride + i] = fmin_ref(1.0f, susc[v]);
 }
 
 // ------------------------------------------------------------ vaccination rollout
@@ -669,15 +694,30 @@ __global__ void __launch_bounds__(256) k_stats(Agents A, Locs L, Tables T, uint3
     const uint32_t nVar = T.nVar;
     uint32_t stride = gridDim.x * blockDim.x;
     unsigned int c[14] = {0};
+    unsigned int varInf[MAXV] = {0}, infV[MAXV] = {0};
+    unsigned long long imm[MAXV] = {0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
-        uint8_t st = A.rec[i].state;
+        const AgentRec r = A.rec[i];
+        uint8_t st = r.state;
         uint8_t w = T.stWB[st];
-        uint8_t fl = A.rec[i].flags;
+        uint8_t fl = r.flags;
         bool inf = T.stInfected[st] != 0;
-        atomicAdd(&sh[S_STATE + st], 1ull);
-        c[S_HOSP] += (w != 4 && w != 3 && ts < A.rec[i].hospUntil);
+        {  // state histogram: one LDS atomic per distinct state of the wave
+            const uint32_t lane = threadIdx.x & 63u;
+            unsigned long long todo = __ballot(true);
+            bool mine = true;
+            while (todo) {
+                const uint32_t first = (uint32_t)(__ffsll((long long)todo) - 1);
+                const uint32_t s0 = (uint32_t)__shfl((int)st, (int)first, 64);
+                const unsigned long long m = __ballot(mine && st == s0);
+                if (lane == first) atomicAdd(&sh[S_STATE + s0], (unsigned long long)__popcll(m));
+                if (st == s0) mine = false;
+                todo &= ~m;
+            }
+        }
+        c[S_HOSP] += (w != 4 && w != 3 && ts < r.hospUntil);
         c[S_STAYED] += (fl & F_STAYED_HOME) ? 1 : 0;
-        bool isQ = A.quarTs[i] <= ts && A.rec[i].quarUntil > ts;
+        bool isQ = A.quarTs[i] <= ts && r.quarUntil > ts;
         c[S_Q] += isQ;
         c[S_QT] += isQ && inf;
         c[S_NQ] += !isQ && inf;
@@ -695,17 +735,24 @@ __global__ void __launch_bounds__(256) k_stats(Agents A, Locs L, Tables T, uint3
         }
         c[S_INFH] += inf && L.rec[A.location[i]].type == T.lt.hospital;
         c[S_VNI] += ic == 0 && mc > 0;
-        if (inf) atomicAdd(&sh[S_VARINF + A.rec[i].variant], 1ull);
         uint8_t sv = A.statVar[i];
-        for (uint32_t v = 0; v < nVar; ++v) {
-            if ((sv >> v) & 1u) atomicAdd(&sh[S_INFV + v], 1ull);
-            float s = A.susc[(size_t)v * A.n + i];
-            long long q = (long long)((double)(1.0f - s) * 16777216.0);
-            if (q) atomicAdd(&sh[S_IMM + v], (unsigned long long)q);
+#pragma unroll
+        for (uint32_t v = 0; v < (uint32_t)MAXV; ++v) {
+            if (v >= nVar) break;
+            varInf[v] += (inf && r.variant == v) ? 1u : 0u;
+            infV[v] += (sv >> v) & 1u;
+            float s = A.susc[(size_t)v * A.suscStride + i];
+            imm[v] += (unsigned long long)(long long)((double)(1.0f - s) * 16777216.0);
         }
     }
     for (int k = 0; k < 14; ++k)
         if (c[k]) atomicAdd(&sh[k], (unsigned long long)c[k]);
+#pragma unroll
+    for (uint32_t v = 0; v < (uint32_t)MAXV; ++v) {
+        if (varInf[v]) atomicAdd(&sh[S_VARINF + v], (unsigned long long)varInf[v]);
+        if (infV[v]) atomicAdd(&sh[S_INFV + v], (unsigned long long)infV[v]);
+        if (imm[v]) atomicAdd(&sh[S_IMM + v], imm[v]);
+    }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < S_COUNT; k += blockDim.x)
         if (sh[k]) atomicAdd(&out[k], sh[k]);
@@ -719,18 +766,6 @@ struct PlanArgs {
     uint64_t seed;
     uint32_t replica, ts, dt, curfewEnd, schoolAge;
 };
-
-// resets of planLocations (movementPolicies.h:1446-1451)
-__global__ void k_plan_reset(PlanArgs a) {
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t i = gid; i < a.A.n; i += stride) {
-        // stepsUntilMove := curfewEnd at this midnight step
-        a.A.wake[i] = a.curfewEnd < 1440u / a.dt ? (uint16_t)a.curfewEnd : WAKE_NONE;
-        a.A.rec[i].flags = (uint8_t)((a.A.rec[i].flags | F_STAYED_HOME) & ~F_NO_WORK);
-    }
-    for (uint32_t l = gid; l < a.L.n; l += stride) a.L.rec[l].noWork = 0u;
-}
 
 // checkSchoolQuarantine (movementPolicies.h:1285-1310), one thread per school
 __global__ void k_plan_school(PlanArgs a) {
@@ -753,44 +788,56 @@ __global__ void k_plan_school(PlanArgs a) {
     }
 }
 
-// checkSchoolWorkQuarantine (movementPolicies.h:1188-1240)
-__global__ void k_plan_schoolwork(PlanArgs a) {
+// Per-agent part of planLocations in one pass: the resets of
+// movementPolicies.h:1446-1451 (stepsUntilMove := curfewEnd, stayedHome,
+// noWorkAgent) and, with quarantinePolicy > 2, checkSchoolWorkQuarantine
+// (movementPolicies.h:1188-1240).
+__global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
     if (i >= A.n) return;
-    uint8_t fl = A.rec[i].flags;
-    if (fl & F_QUARANTINED) return;
-    if (A.rec[i].diagTs > 0 && !a.T.stInfected[A.rec[i].state]) return;
-    const pansim_location_types& lt = a.T.lt;
-    Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
-    uint32_t wl = find_loc(A, a.L, lt, i, lt.work, 0, nullptr);
-    uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
-    uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
-    const LocRec* q = a.L.rec;
-    uint32_t until = 0;
-    if (sl != NOLOC && q[sl].quarUntil > a.ts) until = q[sl].quarUntil;
-    else if (cl != NOLOC && q[cl].quarUntil > a.ts) until = q[cl].quarUntil;
-    else if (wl != NOLOC && q[wl].quarUntil > a.ts) until = q[wl].quarUntil;
-    if (until > 0) {
-        A.rec[i].flags = (uint8_t)(fl | F_QUARANTINED);
-        A.quarTs[i] = a.ts;
-        A.rec[i].quarUntil = until;
-        A.daysInQ[i] += (until - a.ts) / (24u * 60u / a.dt);
+    A.wake[i] = a.curfewEnd < 1440u / a.dt ? (uint16_t)a.curfewEnd : WAKE_NONE;
+    const AgentRec r = A.rec[i];
+    uint8_t fl = (uint8_t)((r.flags | F_STAYED_HOME) & ~F_NO_WORK);
+    if (schoolwork && !(fl & F_QUARANTINED) && !(r.diagTs > 0 && !a.T.stInfected[r.state])) {
+        const pansim_location_types& lt = a.T.lt;
+        Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
+        const CsrReg csr = load_csr(A, r);
+        uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
+        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
+        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
+        const LocRec* q = a.L.rec;
+        uint32_t until = 0;
+        if (sl != NOLOC && q[sl].quarUntil > a.ts) until = q[sl].quarUntil;
+        else if (cl != NOLOC && q[cl].quarUntil > a.ts) until = q[cl].quarUntil;
+        else if (wl != NOLOC && q[wl].quarUntil > a.ts) until = q[wl].quarUntil;
+        if (until > 0) {
+            fl |= F_QUARANTINED;
+            A.quarTs[i] = a.ts;
+            A.rec[i].quarUntil = until;
+            A.daysInQ[i] += (until - a.ts) / (24u * 60u / a.dt);
+        }
     }
+    if (fl != r.flags) A.rec[i].flags = fl;
 }
 
-// quarantine -> closedUntil conversion (movementPolicies.h:1544-1557)
-__global__ void k_plan_close(PlanArgs a) {
+// Per-location part: noWork reset (movementPolicies.h:1449) and, with
+// quarantinePolicy > 2, the quarantine -> closedUntil conversion
+// (movementPolicies.h:1544-1557).
+__global__ void k_plan_locs(PlanArgs a, uint32_t close) {
     uint32_t stride = gridDim.x * blockDim.x;
     const pansim_location_types& lt = a.T.lt;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < a.L.n; l += stride) {
-        uint32_t q = a.L.rec[l].quarUntil;
-        uint32_t t = a.L.rec[l].type;
+        LocRec* lr = &a.L.rec[l];
+        lr->noWork = 0u;
+        if (!close) continue;
+        uint32_t q = lr->quarUntil;
+        uint32_t t = lr->type;
         if (q > a.ts && (t == lt.school || t == lt.classroom)) {
-            a.L.rec[l].closedUntil = q;
-            a.L.rec[l].quarUntil = 0;
+            lr->closedUntil = q;
+            lr->quarUntil = 0;
         } else if (q > a.ts && t == lt.work) {
-            a.L.rec[l].quarUntil = 0;
+            lr->quarUntil = 0;
         }
     }
 }
@@ -808,14 +855,15 @@ __global__ void k_plan_underage(PlanArgs a) {
     if (A.rec[i].flags & F_QUARANTINED) flag = true;
     else {
         Stream rs(a.seed, i, a.ts, PH_PLAN_UNDERAGE, a.replica);
-        uint32_t sl = find_loc(A, a.L, lt, i, lt.school, 0, nullptr);
-        uint32_t cl = find_loc(A, a.L, lt, i, lt.classroom, 0, &rs);
+        const CsrReg csr = load_csr(A, A.rec[i]);
+        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
+        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
         bool tooOld = A.rec[i].age >= a.schoolAge;
         flag = tooOld || (sl != NOLOC && (a.L.rec[sl].open == 0 || a.L.rec[sl].closedUntil > a.ts)) ||
                (cl != NOLOC && (a.L.rec[cl].open == 0 || a.L.rec[cl].closedUntil > a.ts));
     }
     if (flag) {
-        uint32_t h = find_loc(A, a.L, lt, i, lt.home, 0, nullptr);
+        uint32_t h = A.rec[i].home;
         if (h != NOLOC) a.L.rec[h].noWork = NOWORK_FLAGGED;
     }
 }
@@ -827,7 +875,7 @@ __global__ void k_plan_nowork_claim(PlanArgs a) {
     const Agents& A = a.A;
     if (i >= A.n) return;
     if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
-    uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
+    uint32_t h = A.rec[i].home;
     // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent + 1)
     if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, i + 1u);
 }
@@ -837,7 +885,7 @@ __global__ void k_plan_nowork_resolve(PlanArgs a) {
     const Agents& A = a.A;
     if (i >= A.n) return;
     if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
-    uint32_t h = find_loc(A, a.L, a.T.lt, i, a.T.lt.home, 0, nullptr);
+    uint32_t h = A.rec[i].home;
     if (h != NOLOC && a.L.rec[h].noWork == i + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
 }
 
@@ -862,7 +910,7 @@ __global__ void k_expose_mutation(Agents A, Tables T, uint64_t seed, uint32_t re
                                   double fraction) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    float s = A.susc[(size_t)v * A.n + i];
+    float s = A.susc[(size_t)v * A.suscStride + i];
     if (!(s > 0.0f)) return;
     Stream rs(seed, i, ts, PH_MUTATION, replica);
     if (rs.unit() < fraction * (double)s) {

@@ -281,6 +281,8 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
     TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));
+    TRY(dalloc(c, &L.visited, ((size_t)NL + 31) / 32));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.visited, 0, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &L.pendKey, NL));
     {
         std::vector<float> ones(HAZARD_TYPES, 1.0f);
@@ -327,7 +329,13 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(upload(c, &A.sex, a->sex, N));
     TRY(upload(c, &A.precond, a->precond, N));
     TRY(dalloc(c, &A.days, N));
-    TRY(upload(c, &A.location, a->location, N));
+    // location and susceptibility arrays padded to whole infection tiles
+    // (16-byte vector loads); the padding holds location 0 and susceptibility 0
+    const size_t nPad = ((size_t)N + INFECT_TILE - 1) / INFECT_TILE * INFECT_TILE;
+    A.suscStride = (uint32_t)nPad;
+    TRY(dalloc(c, &A.location, nPad));
+    HIP_OR_FAIL(c, hipMemsetAsync(A.location, 0, nPad * sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(A.location, a->location, (size_t)N * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     {
         // padded to whole movement tiles with WAKE_NONE: the scan reads 16-byte vectors
         const size_t nw = ((size_t)N + MOVE_TILE - 1) / MOVE_TILE * MOVE_TILE;
@@ -336,7 +344,8 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     }
     TRY(dalloc(c, &A.lastTest, N));
     TRY(upload(c, &A.scal, a->scaling_age_sex, N));
-    TRY(dalloc(c, &A.susc, (size_t)V * N));
+    TRY(dalloc(c, &A.susc, (size_t)V * nPad));
+    HIP_OR_FAIL(c, hipMemsetAsync(A.susc, 0, (size_t)V * nPad * sizeof(float), c->stream));
     TRY(dalloc(c, &A.slot6, (size_t)V * N));
     uint32_t** u32s[] = {&A.infTs, &A.infLoc, &A.worstTs, &A.worstEndTs, &A.quarTs, &A.daysInQ, &A.immTs, &A.recTs};
     for (auto p : u32s) TRY(dalloc(c, p, N));
@@ -524,7 +533,8 @@ int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, floa
     InfectArgs a = infect_args(c, timestamp, mask, k, season, dt);
     unsigned grid = blocks_for(c->A.n, INFECT_TILE);
     if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
-    hipLaunchKernelGGL(k_infect, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
+    if (a.hazard) hipLaunchKernelGGL(k_infect<true>, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_infect<false>, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
     if (c->timing) {
         if (record_event(c, c->infectEv, 2 * c->infectN + 1)) return PANSIM_E_DEVICE;
         c->infectN++;
@@ -558,13 +568,11 @@ int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
     const unsigned gN = blocks_for(c->A.n);
     if (p->time_step != c->dt) return fail(c, PANSIM_E_ARG, "plan_locations: time step differs from the one given at creation");
     c->wakeDay = p->timestamp - p->timestamp % (1440u / c->dt);
-    hipLaunchKernelGGL(k_plan_reset, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a);
     if (p->quarantine_policy > 3 && c->L.nSchools)
         hipLaunchKernelGGL(k_plan_school, dim3(blocks_for(c->L.nSchools)), dim3(256), 0, c->stream, a);
-    if (p->quarantine_policy > 2) {
-        hipLaunchKernelGGL(k_plan_schoolwork, dim3(gN), dim3(256), 0, c->stream, a);
-        hipLaunchKernelGGL(k_plan_close, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a);
-    }
+    const uint32_t qp2 = p->quarantine_policy > 2 ? 1u : 0u;
+    hipLaunchKernelGGL(k_plan_agents, dim3(gN), dim3(256), 0, c->stream, a, qp2);
+    hipLaunchKernelGGL(k_plan_locs, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a, qp2);
     hipLaunchKernelGGL(k_plan_underage, dim3(gN), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_plan_nowork_claim, dim3(gN), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_plan_nowork_resolve, dim3(gN), dim3(256), 0, c->stream, a);
@@ -1173,7 +1181,10 @@ int pansim_batch_step(pansim_batch* b, uint32_t timestamp) {
     a.pendPar = b->pendPar;
     hipLaunchKernelGGL(k_move_batch, dim3(b->moveBlocks), dim3(MOVE_BLOCK), 0, b->stream, a);
     hipLaunchKernelGGL(k_resolve_batch, dim3(n * RESOLVE_BLOCKS_PER_REPLICA), dim3(256), 0, b->stream, a);
-    hipLaunchKernelGGL(k_infect_batch, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
+    bool anyHazard = false;
+    for (pansim_ctx* c : b->reps) anyHazard = anyHazard || c->hazardOn;
+    if (anyHazard) hipLaunchKernelGGL(k_infect_batch<true>, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
+    else hipLaunchKernelGGL(k_infect_batch<false>, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
     b->pendPar ^= 1u;
     for (pansim_ctx* c : b->reps) c->pendPar = b->pendPar;
     hipError_t e = hipGetLastError();

@@ -91,7 +91,8 @@ struct Agents {
     uint16_t* wake;  // step of the day of the next full movement evaluation, WAKE_NONE if none today
     uint32_t* lastTest;
     float* scal;   // scalingAgeSex
-    float* susc;   // [V][n]
+    float* susc;   // [V][suscStride]
+    uint32_t suscStride;  // n rounded up to whole infection tiles (padding: 0)
     float* slot6;  // [V][n] scalingSymptoms[v*7+6]
     // AgentStats not in AgentRec
     uint32_t* infTs;
@@ -118,12 +119,17 @@ struct Locs {
     LocRec* rec;
     LocDyn* dyn;
     long long* sumX;                // [V-1][n]: sums of variants 1..V-1
+    uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
     unsigned long long* pendKey;    // [n]
     const uint32_t* schools;
     const uint32_t* classrooms;
     const uint32_t* classOff;
     uint32_t nSchools;
 };
+
+// Marks location l as visited by an infectious agent (bits are only cleared
+// by the full rebuild at midnight, so a set bit may have a zero sum again).
+__device__ __forceinline__ void mark_visited(const Locs& L, uint32_t l) { atomicOr(&L.visited[l >> 5], 1u << (l & 31u)); }
 
 // Address of the exact infectiousness sum of variant v at location l.
 __device__ __forceinline__ long long* loc_sum(const Locs& L, uint32_t v, uint32_t l) {

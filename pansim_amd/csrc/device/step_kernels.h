@@ -577,6 +577,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         long long q = fixed32(x);
         atomicAdd((unsigned long long*)loc_sum(L, var, oldLoc), (unsigned long long)(-q));
         atomicAdd((unsigned long long*)loc_sum(L, var, loc), (unsigned long long)q);
+        mark_visited(L, loc);
     }
     atomicSub(&L.dyn[oldLoc].count, 1u);
     atomicAdd(&L.dyn[loc].count, 1u);
@@ -683,11 +684,12 @@ struct InfectArgs {
 // Infection of all active variants.  LocationsList::infectAgents
 // (locationList.h:251-305) with the ratio of infectionPolicies.h:523-546
 // evaluated from the location's exact sums.  Each thread takes INFECT_ITEMS
-// agents (stride INFECT_BLOCK) and issues their independent loads together,
-// so that a wave keeps several memory round trips in flight.
+// consecutive agents and issues their independent loads together (16-byte
+// vectors), so that a wave keeps several memory round trips in flight.
 constexpr uint32_t INFECT_BLOCK = 256;
-constexpr uint32_t INFECT_ITEMS = 8;
+constexpr uint32_t INFECT_ITEMS = 8;  // two 16-byte vectors of agents per thread
 constexpr uint32_t INFECT_TILE = INFECT_BLOCK * INFECT_ITEMS;
+static_assert(INFECT_ITEMS == 8, "the infection pass loads two 4-agent vectors per thread");
 
 // The expected-infection hazard of one susceptible agent at a location with
 // infectious visitors, s * (1 - exp(-x)) with x = k * density * inf * m_type *
@@ -702,6 +704,7 @@ __device__ __forceinline__ void hazard_terms(double* sh, float k, float density,
     atomicAdd(&sh[2 + type], w * ((double)k * (double)density * (double)inf * (double)season * (double)dt));
 }
 
+template <bool HAZARD>
 __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
                                              uint32_t count, float inf, float s, double* sh) {
     const uint32_t n = a.A.n;
@@ -711,7 +714,7 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
     const float m = a.typeMul[type];
     const float infm = inf * m;
     const float x = a.k * density * infm * a.season * a.dt;
-    if (sh) hazard_terms(sh, a.k, density, inf, m, type, a.season, a.dt, x, s);
+    if (HAZARD) hazard_terms(sh, a.k, density, inf, m, type, a.season, a.dt, x, s);
     float ratio = 1.0f - ps_expf(-x);
     if (ratio == 0.0f) return;
     Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
@@ -722,7 +725,7 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
         a.A.rec[i].variant = (uint8_t)v;
         a.A.days[i] = -2;
         float sf = a.T.stSusc[fi] ? 1.0f : 0.0f;
-        for (uint32_t u = 0; u < a.T.nVar; ++u) a.A.susc[(size_t)u * n + i] = sf;
+        for (uint32_t u = 0; u < a.T.nVar; ++u) a.A.susc[(size_t)u * a.A.suscStride + i] = sf;
         a.A.infTs[i] = a.timestamp;
         a.A.infCount[i] = (uint8_t)(a.A.infCount[i] + 1);
         a.A.infLoc[i] = loc;
@@ -732,58 +735,73 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
     }
 }
 
+// HAZARD: accumulate the expected-infection hazard and its gradient
+// (a separate instantiation, so that the plain pass carries no FP64 state).
+template <bool HAZARD>
 __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) {
     if (a.mask == 0u) return;
-    __shared__ double shHazard[HAZARD_SLOTS];
-    double* sh = nullptr;
-    if (a.hazard) {  // uniform over the grid
+    __shared__ double shHazard[HAZARD ? HAZARD_SLOTS : 1];
+    double* sh = shHazard;
+    if (HAZARD) {
         for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
         __syncthreads();
-        sh = shHazard;
     }
-    const uint32_t n = a.A.n;
-    const uint32_t base = tile * INFECT_TILE + threadIdx.x;
+    // thread t takes agents base + 8t .. 8t + 7 with 16-byte loads (the
+    // location and susceptibility arrays are padded to whole tiles)
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const uint32_t base = tile * INFECT_TILE + threadIdx.x * INFECT_ITEMS;
     uint32_t loc[INFECT_ITEMS];
-#pragma unroll
-    for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
-        uint32_t i = base + k * INFECT_BLOCK;
-        loc[k] = i < n ? __builtin_nontemporal_load(&a.A.location[i]) : NOLOC;
+    {
+        const v4u* lv = reinterpret_cast<const v4u*>(a.A.location + base);
+        const v4u l0 = __builtin_nontemporal_load(&lv[0]), l1 = __builtin_nontemporal_load(&lv[1]);
+        loc[0] = l0.x; loc[1] = l0.y; loc[2] = l0.z; loc[3] = l0.w;
+        loc[4] = l1.x; loc[5] = l1.y; loc[6] = l1.z; loc[7] = l1.w;
     }
     for (uint32_t v = 0; v < a.T.nVar; ++v) {
         if (!((a.mask >> v) & 1u)) continue;
         // susceptibility is re-read per variant: an infection by an earlier
-        // variant in this launch has already rewritten it
+        // variant in this launch has already rewritten it (padding reads 0)
         float s[INFECT_ITEMS];
-#pragma unroll
-        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
-            uint32_t i = base + k * INFECT_BLOCK;
-            s[k] = i < n ? a.A.susc[(size_t)v * n + i] : 0.0f;
+        {
+            const v4f* sv4 = reinterpret_cast<const v4f*>(a.A.susc + (size_t)v * a.A.suscStride + base);
+            const v4f s0 = sv4[0], s1 = sv4[1];
+            s[0] = s0.x; s[1] = s0.y; s[2] = s0.z; s[3] = s0.w;
+            s[4] = s1.x; s[5] = s1.y; s[6] = s1.z; s[7] = s1.w;
         }
-        // one 8-byte gather per susceptible agent: the location's sum of this variant
+        // the visited bit of each susceptible agent's location (an L2-resident
+        // bitmap), then the location's sum of this variant where it is set
+        uint32_t vis[INFECT_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) vis[k] = s[k] > 0.0f ? a.L.visited[loc[k] >> 5] : 0u;
         const long long* sv = v == 0 ? &a.L.dyn[0].sum0 : a.L.sumX + (size_t)(v - 1) * a.L.n;
         const uint32_t pitch = v == 0 ? (uint32_t)(sizeof(LocDyn) / sizeof(long long)) : 1u;
         uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
-            if (s[k] > 0.0f && sv[(size_t)loc[k] * pitch] != 0) hits |= 1u << k;
+            if (((vis[k] >> (loc[k] & 31u)) & 1u) && sv[(size_t)loc[k] * pitch] != 0) hits |= 1u << k;
         // the rare agents at a location with infectious visitors: one at a time
         while (hits) {
             const uint32_t k = (uint32_t)__builtin_ctz(hits);
             hits &= hits - 1u;
-            const uint32_t i = base + k * INFECT_BLOCK;
+            const uint32_t i = base + k;
             const uint32_t l = a.A.location[i];
             const LocDyn d = a.L.dyn[l];
-            infect_agent(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, a.A.susc[(size_t)v * n + i], sh);
+            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf,
+                                 a.A.susc[(size_t)v * a.A.suscStride + i], sh);
         }
     }
-    if (sh) {
+    if (HAZARD) {
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x)
             if (shHazard[j] != 0.0) atomicAdd(&a.hazard[j], shHazard[j]);
     }
 }
 
-__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) { infect_tile(a, blockIdx.x); }
+template <bool HAZARD>
+__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
+    infect_tile<HAZARD>(a, blockIdx.x);
+}
 
 // ------------------------------------------------------------ replica batches
 // Many populations (replicas) stepped by one launch per kernel: every block
@@ -820,11 +838,15 @@ __global__ void __launch_bounds__(256) k_resolve_batch(BatchArgs b) {
     resolve_pending(m.L, m.S, b.pendPar, blockIdx.x % RESOLVE_BLOCKS_PER_REPLICA, RESOLVE_BLOCKS_PER_REPLICA);
 }
 
+template <bool HAZARD>
 __global__ void __launch_bounds__(INFECT_BLOCK) k_infect_batch(BatchArgs b) {
     const uint32_t r = b.infRep[blockIdx.x];
     InfectArgs a = b.infect[r];
     a.timestamp = b.ts;
-    infect_tile(a, blockIdx.x - b.infFirst[r]);
+    // a replica without hazard accumulation in a batch that has some: its
+    // blocks take the plain path
+    if (HAZARD && a.hazard) infect_tile<true>(a, blockIdx.x - b.infFirst[r]);
+    else infect_tile<false>(a, blockIdx.x - b.infFirst[r]);
 }
 
 }  // namespace dev
