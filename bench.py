@@ -14,8 +14,9 @@ every agent.  --warmup steps run untimed first (default: day 0), then exactly
 --steps steps are timed between two device synchronisations (default: the
 remaining 27 days).  value = agents * steps / seconds (agent-steps/s).
 
-roofline: the per-step pipeline (movement kernel, quarantine-write resolve,
-infection kernel) is the dominant work; its algorithmic bytes per step are
+roofline: the per-step pipeline (movement kernel, then the infection kernel,
+which also applies the movement's deferred location quarantine writes) is
+the dominant work; its algorithmic bytes per step are
 22 B/agent + 24 B/location + 4 B/relocated agent, divided by its device time
 from HIP events recorded around every movement and infection call on the
 context stream during the timed region.  traffic: HBM bytes per step of the

@@ -37,6 +37,8 @@ struct pansim_ctx {
     size_t moveN = 0, infectN = 0;
     unsigned long long* moveCounters = nullptr;  // MOVE_CTR_SLOTS pairs (evaluated movers, relocated agents)
     uint32_t pendPar = 0;                         // parity of the pending-write counter of the next step
+    bool resolvePending = false;                  // a movement step's quarantine writes are not applied yet
+    uint32_t resolvePar = 0;
     // schedule storage (re-uploadable)
     uint32_t* evOffDev = nullptr;
     pansim_event* evDev = nullptr;
@@ -111,6 +113,16 @@ int check_launch(pansim_ctx* c, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, PANSIM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
     return 0;
+}
+
+// Applies a movement step's deferred quarantine writes when no infection
+// launch has taken them (every entry point but pansim_infection_step calls
+// this first, so the location state it sees is complete).
+static int flush_resolve(pansim_ctx* c) {
+    if (!c->resolvePending) return 0;
+    c->resolvePending = false;
+    hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, c->resolvePar);
+    return check_launch(c, "resolve_pending");
 }
 
 unsigned persistent_blocks(pansim_ctx* c) { return (unsigned)c->numCUs * 8u; }
@@ -405,6 +417,7 @@ void pansim_destroy(pansim_ctx* c) {
 
 int pansim_sync(pansim_ctx* c) {
     if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -508,11 +521,16 @@ static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, 
     a.dt = (float)dt;
     a.typeMul = c->typeMul;
     a.hazard = c->hazardOn ? c->hazard : nullptr;
+    a.resolve = 0;
+    a.resolvePar = 0;
+    a.nTiles = blocks_for(c->A.n, INFECT_TILE);
     return a;
 }
 
+
 int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     if (!c || !p) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     if (p->time_step != c->dt) return fail(c, PANSIM_E_ARG, "movement_step: time step differs from the one given at creation");
     MoveArgs m = move_args(c, p);
     if (p->timestamp - m.stepOfDay != c->wakeDay)
@@ -520,7 +538,8 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     c->pendPar ^= 1u;
     if (c->timing && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
-    hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, m.pendPar);
+    c->resolvePending = true;
+    c->resolvePar = m.pendPar;
     if (c->timing) {
         if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
         c->moveN++;
@@ -531,6 +550,9 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
 int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, float k, float season, uint32_t dt) {
     if (!c) return PANSIM_E_ARG;
     InfectArgs a = infect_args(c, timestamp, mask, k, season, dt);
+    a.resolve = c->resolvePending ? 1u : 0u;
+    a.resolvePar = c->resolvePar;
+    c->resolvePending = false;
     unsigned grid = blocks_for(c->A.n, INFECT_TILE);
     if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
     if (a.hazard) hipLaunchKernelGGL(k_infect<true>, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
@@ -550,11 +572,13 @@ static int rebuild_locations(pansim_ctx* c) {
 
 int pansim_rebuild_locations(pansim_ctx* c) {
     if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     return rebuild_locations(c);
 }
 
 int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
     if (!c || !p) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     PlanArgs a;
     a.A = c->A;
     a.L = c->L;
@@ -584,6 +608,7 @@ int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
 int pansim_testing_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const double probs[6], uint32_t delay, uint32_t qlen,
                        uint32_t usePCR, uint32_t out[3]) {
     if (!c || !probs) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     TestArgs a;
     a.A = c->A;
     a.L = c->L;
@@ -618,6 +643,7 @@ int pansim_testing_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const double pro
 
 int pansim_progression_day(pansim_ctx* c, uint32_t ts, uint32_t dt) {
     if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     hipLaunchKernelGGL(k_progression, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
                        ts, dt);
     return check_launch(c, "progression");
@@ -626,6 +652,7 @@ int pansim_progression_day(pansim_ctx* c, uint32_t ts, uint32_t dt) {
 int pansim_immunization_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const float prot[60], const int32_t sim[7],
                             const float simMul[7]) {
     if (!c || !prot || !sim || !simMul) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     ImmArgs a;
     a.A = c->A;
     a.T = c->T;
@@ -648,6 +675,7 @@ int pansim_immunization_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const float
 
 int pansim_vaccination_init(pansim_ctx* c, const int32_t order[10], const float group_level[10]) {
     if (!c || !order || !group_level) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     // the willingness of group g is vaccinationGroupLevel[levelOf[g]], as the
     // reference's category lambdas capture it (immunization.h:238-322)
     static const int levelOf[10] = {0, 1, 3, 4, 5, 7, 2, 6, 8, 9};
@@ -671,6 +699,7 @@ int pansim_vaccination_init(pansim_ctx* c, const int32_t order[10], const float 
 
 int pansim_age_group_counts(pansim_ctx* c, uint32_t out[10]) {
     if (!c || !out) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, 10 * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_age_groups, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->A, c->counters);
     int r = check_launch(c, "age_group_counts");
@@ -699,6 +728,7 @@ static VaccDayArgs vacc_day_args(pansim_ctx* c, uint32_t ts, uint32_t dt) {
 int pansim_booster_round(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t round, uint32_t available,
                          const float age_group_frac[10]) {
     if (!c || !age_group_frac || dt == 0) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     if (available == 0) return 0;
     VaccDayArgs a = vacc_day_args(c, ts, dt);
     a.round = round;
@@ -722,6 +752,7 @@ int pansim_booster_round(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t round
 int pansim_vaccinate(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t available, uint32_t* current_category,
                      uint32_t* immunized_today) {
     if (!c || !current_category || dt == 0) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     VaccDayArgs a = vacc_day_args(c, ts, dt);
     uint32_t& cat = *current_category;
     auto count_round = [&](uint32_t cc, uint32_t any, uint32_t* out) -> int {
@@ -769,6 +800,7 @@ int pansim_vaccinate(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t available
 
 int pansim_other_disease_day(pansim_ctx* c, uint32_t ts, uint32_t dt, float probabilityMul) {
     if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     hipLaunchKernelGGL(k_other_disease, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed,
                        c->replica, ts, dt, probabilityMul);
     return check_launch(c, "other_disease");
@@ -776,6 +808,7 @@ int pansim_other_disease_day(pansim_ctx* c, uint32_t ts, uint32_t dt, float prob
 
 int pansim_count_hospitalized_covid(pansim_ctx* c, uint32_t ts, uint32_t dueWith, uint32_t* out) {
     if (!c || !out) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_covid_hospital, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->L, c->T, ts,
                        dueWith, c->counters);
@@ -789,6 +822,7 @@ int pansim_count_hospitalized_covid(pansim_ctx* c, uint32_t ts, uint32_t dueWith
 
 int pansim_count_variants(pansim_ctx* c, uint32_t* out) {
     if (!c || !out) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, MAXV * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_variants, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->counters);
     int r = check_launch(c, "count_variants");
@@ -803,6 +837,7 @@ int pansim_count_variants(pansim_ctx* c, uint32_t* out) {
 int pansim_stats_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const uint32_t test[3], uint32_t immunizedToday,
                      uint32_t* out, size_t cap, size_t* len) {
     if (!c || !test) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, S_COUNT * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_stats, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->A, c->L, c->T, ts, dt, c->counters);
     int r = check_launch(c, "stats");
@@ -851,6 +886,7 @@ int pansim_stats_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const uint32_t tes
 
 int pansim_set_open_by_type(pansim_ctx* c, const uint32_t* types, uint32_t n, uint32_t open) {
     if (!c || (n && !types) || n > 16) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     uint32_t* dtypes = (uint32_t*)(c->counters + S_COUNT);  // 16 u64 words of scratch
     HIP_OR_FAIL(c, hipMemcpyAsync(dtypes, types, n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_set_open, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, dtypes, n, (uint8_t)(open ? 1 : 0));
@@ -862,12 +898,14 @@ int pansim_set_open_by_type(pansim_ctx* c, const uint32_t* types, uint32_t n, ui
 
 int pansim_scale_infectiousness(pansim_ctx* c, double factor, uint32_t divide) {
     if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     hipLaunchKernelGGL(k_scale_inf, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.lt.home, factor, divide);
     return check_launch(c, "scale_infectiousness");
 }
 
 int pansim_expose_mutation(pansim_ctx* c, uint32_t ts, uint32_t v, double fraction) {
     if (!c || v >= c->T.nVar) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     hipLaunchKernelGGL(k_expose_mutation, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
                        ts, v, fraction);
     return check_launch(c, "expose_mutation");
@@ -875,6 +913,7 @@ int pansim_expose_mutation(pansim_ctx* c, uint32_t ts, uint32_t v, double fracti
 
 int pansim_upload_schedules(pansim_ctx* c, const uint32_t* off, uint32_t nOff, const pansim_event* ev, uint32_t nEv) {
     if (!c || !off || nOff != c->evOffCap) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     if (nEv > c->evCap) return fail(c, PANSIM_E_ARG, "schedule grew beyond its reserved capacity");
     if (!offsets_ok(off, nOff, nEv)) return fail(c, PANSIM_E_ARG, "schedule offsets out of range");
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
@@ -887,6 +926,7 @@ int pansim_upload_schedules(pansim_ctx* c, const uint32_t* off, uint32_t nOff, c
 
 int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_t n) {
     if (!c || !out || v >= c->T.nVar || n < c->L.n) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     if (v == 0)
         HIP_OR_FAIL(c, hipMemcpy2D(out, sizeof(int64_t), &c->L.dyn[0].sum0, sizeof(LocDyn), sizeof(int64_t), c->L.n,
@@ -898,6 +938,7 @@ int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_
 
 int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t* out, size_t n) {
     if (!c || !name || !out) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     const uint32_t N = c->A.n, NL = c->L.n;
     std::string f = name;
@@ -1054,6 +1095,7 @@ int pansim_hazard_enable(pansim_ctx* c, uint32_t on) {
 
 int pansim_hazard_read(pansim_ctx* c, double* out, uint32_t n, uint32_t reset) {
     if (!c || !out || n < HAZARD_SLOTS) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     HIP_OR_FAIL(c, hipMemcpy(out, c->hazard, HAZARD_SLOTS * sizeof(double), hipMemcpyDeviceToHost));
     if (reset) HIP_OR_FAIL(c, hipMemset(c->hazard, 0, HAZARD_SLOTS * sizeof(double)));
@@ -1107,7 +1149,8 @@ int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_
     std::vector<uint32_t> moveRep, moveFirst(n), infRep, infFirst(n);
     for (uint32_t r = 0; r < n; ++r) {
         pansim_ctx* c = replicas[r];
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(PANSIM_E_DEVICE, "pansim_batch_create: sync failed");
+        if (flush_resolve(c) != 0 || hipStreamSynchronize(c->stream) != hipSuccess)
+            return bail(PANSIM_E_DEVICE, "pansim_batch_create: sync failed");
         b->reps.push_back(c);
         b->ownStreams.push_back(c->stream);
         c->stream = b->stream;
@@ -1180,7 +1223,6 @@ int pansim_batch_step(pansim_batch* b, uint32_t timestamp) {
     a.stepOfDay = stepOfDay;
     a.pendPar = b->pendPar;
     hipLaunchKernelGGL(k_move_batch, dim3(b->moveBlocks), dim3(MOVE_BLOCK), 0, b->stream, a);
-    hipLaunchKernelGGL(k_resolve_batch, dim3(n * RESOLVE_BLOCKS_PER_REPLICA), dim3(256), 0, b->stream, a);
     bool anyHazard = false;
     for (pansim_ctx* c : b->reps) anyHazard = anyHazard || c->hazardOn;
     if (anyHazard) hipLaunchKernelGGL(k_infect_batch<true>, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);

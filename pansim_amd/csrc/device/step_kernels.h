@@ -679,6 +679,9 @@ struct InfectArgs {
     float dt;
     const float* typeMul;  // [HAZARD_TYPES] infectiousness multiplier of each location type (1 = reference)
     double* hazard;        // [HAZARD_SLOTS] accumulated expected infections and gradient, or null
+    uint32_t resolve;      // also apply the preceding movement step's quarantine writes
+    uint32_t resolvePar;   // their pending-counter parity
+    uint32_t nTiles;       // infection blocks of this population (for the resolve stride)
 };
 
 // Infection of all active variants.  LocationsList::infectAgents
@@ -739,6 +742,9 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
 // (a separate instantiation, so that the plain pass carries no FP64 state).
 template <bool HAZARD>
 __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) {
+    // the movement step's deferred location writes: the infection pass never
+    // reads quarantine state, so they share its launch
+    if (a.resolve) resolve_pending(a.L, a.S, a.resolvePar, tile, a.nTiles);
     if (a.mask == 0u) return;
     __shared__ double shHazard[HAZARD ? HAZARD_SLOTS : 1];
     double* sh = shHazard;
@@ -808,8 +814,6 @@ __global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
 // finds its replica in a per-block table and runs the same tile code on that
 // replica's arrays.  The per-day parameters of every replica live in device
 // arrays; the step's timestamp is a kernel argument, shared by all replicas.
-constexpr uint32_t RESOLVE_BLOCKS_PER_REPLICA = 4;
-
 struct BatchArgs {
     const MoveArgs* move;       // [replicas]
     const InfectArgs* infect;   // [replicas]
@@ -832,17 +836,13 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_batch(BatchArgs b) {
     move_tile(m, blockIdx.x - b.moveFirst[r]);
 }
 
-__global__ void __launch_bounds__(256) k_resolve_batch(BatchArgs b) {
-    const uint32_t r = blockIdx.x / RESOLVE_BLOCKS_PER_REPLICA;
-    const MoveArgs& m = b.move[r];
-    resolve_pending(m.L, m.S, b.pendPar, blockIdx.x % RESOLVE_BLOCKS_PER_REPLICA, RESOLVE_BLOCKS_PER_REPLICA);
-}
-
 template <bool HAZARD>
 __global__ void __launch_bounds__(INFECT_BLOCK) k_infect_batch(BatchArgs b) {
     const uint32_t r = b.infRep[blockIdx.x];
     InfectArgs a = b.infect[r];
     a.timestamp = b.ts;
+    a.resolve = 1u;  // every batch step is a movement step followed by this launch
+    a.resolvePar = b.pendPar;
     // a replica without hazard accumulation in a batch that has some: its
     // blocks take the plain path
     if (HAZARD && a.hazard) infect_tile<true>(a, blockIdx.x - b.infFirst[r]);
