@@ -277,12 +277,20 @@ int pansim_download_field(pansim_ctx* ctx, const char* name, uint32_t timestamp,
 /* Per-location infectiousness sum of one variant, 2^-32 fixed point. */
 int pansim_download_location_sums(pansim_ctx* ctx, uint32_t variant, int64_t* out, size_t n);
 
-/* Device timing: while enabled, every pansim_movement_step and
- * pansim_infection_step is bracketed by HIP events on the context stream.
- * pansim_enable_timing resets the record; pansim_timing_summary returns the
- * number of bracketed calls and their summed device time in milliseconds;
- * pansim_last_kernel_ms the time of the last call of each. */
-int pansim_enable_timing(pansim_ctx* ctx, uint32_t on);
+/* Device timing with HIP events on the context stream.
+ * PANSIM_TIMING_KERNELS: every pansim_movement_step and pansim_infection_step
+ *   is bracketed by its own event pair; pansim_timing_summary returns the
+ *   number of bracketed calls of each and their summed device time in
+ *   milliseconds, pansim_last_kernel_ms the time of the last call of each.
+ * PANSIM_TIMING_PIPELINE: one event pair per step, from the movement launch to
+ *   the end of the infection launch that follows it (half the events of the
+ *   kernel mode); pansim_timing_summary returns the pair count and summed time
+ *   as n_move / move_ms and zero infection calls.
+ * pansim_enable_timing resets the record. */
+#define PANSIM_TIMING_OFF 0u
+#define PANSIM_TIMING_KERNELS 1u
+#define PANSIM_TIMING_PIPELINE 2u
+int pansim_enable_timing(pansim_ctx* ctx, uint32_t mode);
 int pansim_timing_summary(pansim_ctx* ctx, uint32_t* n_move, double* move_ms, uint32_t* n_infect, double* infect_ms);
 int pansim_last_kernel_ms(pansim_ctx* ctx, float* move_ms, float* infect_ms);
 

@@ -31,10 +31,12 @@ struct pansim_ctx {
     unsigned long long* hostCounters = nullptr;
     uint32_t hcwCount = 0;
     int numCUs = 256;
-    bool timing = false;
-    // per-call HIP event pairs on the context stream while timing is on
+    uint32_t timing = 0;  // PANSIM_TIMING_* mode
+    // per-call HIP event pairs on the context stream (kernel mode), or one
+    // pair per movement + infection step (pipeline mode, in moveEv)
     std::vector<hipEvent_t> moveEv, infectEv;
     size_t moveN = 0, infectN = 0;
+    bool pipeOpen = false;  // pipeline mode: a movement started a pair
     unsigned long long* moveCounters = nullptr;  // MOVE_CTR_SLOTS pairs (evaluated movers, relocated agents)
     uint32_t pendPar = 0;                         // parity of the pending-write counter of the next step
     bool resolvePending = false;                  // a movement step's quarantine writes are not applied yet
@@ -117,8 +119,10 @@ int check_launch(pansim_ctx* c, const char* what) {
 
 // Applies a movement step's deferred quarantine writes when no infection
 // launch has taken them (every entry point but pansim_infection_step calls
-// this first, so the location state it sees is complete).
+// this first, so the location state it sees is complete), and ends an open
+// pipeline timing pair.
 static int flush_resolve(pansim_ctx* c) {
+    c->pipeOpen = false;  // other work between a movement and its infection: not timed as a pipeline step
     if (!c->resolvePending) return 0;
     c->resolvePending = false;
     hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, c->resolvePar);
@@ -425,15 +429,19 @@ int pansim_sync(pansim_ctx* c) {
 int pansim_enable_timing(pansim_ctx* c, uint32_t on) {
     if (!c) return PANSIM_E_ARG;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    c->timing = on != 0;
+    if (on > PANSIM_TIMING_PIPELINE) return fail(c, PANSIM_E_ARG, "enable_timing: unknown mode");
+    c->timing = on;
     c->moveN = c->infectN = 0;
+    c->pipeOpen = false;
     return 0;
 }
 
+// Timing events: timestamps only, so no system-scope fence (cache
+// writeback) at each record.
 static int record_event(pansim_ctx* c, std::vector<hipEvent_t>& v, size_t idx) {
     while (v.size() <= idx) {
         hipEvent_t e;
-        HIP_OR_FAIL(c, hipEventCreate(&e));
+        HIP_OR_FAIL(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         v.push_back(e);
     }
     HIP_OR_FAIL(c, hipEventRecord(v[idx], c->stream));
@@ -536,14 +544,15 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     if (p->timestamp - m.stepOfDay != c->wakeDay)
         return fail(c, PANSIM_E_ARG, "movement_step: pansim_plan_locations must run at the start of every day");
     c->pendPar ^= 1u;
-    if (c->timing && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+    if (c->timing != PANSIM_TIMING_OFF && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
     c->resolvePending = true;
     c->resolvePar = m.pendPar;
-    if (c->timing) {
+    if (c->timing == PANSIM_TIMING_KERNELS) {
         if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
         c->moveN++;
     }
+    c->pipeOpen = c->timing == PANSIM_TIMING_PIPELINE;
     return check_launch(c, "movement");
 }
 
@@ -553,13 +562,17 @@ int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, floa
     a.resolve = c->resolvePending ? 1u : 0u;
     a.resolvePar = c->resolvePar;
     c->resolvePending = false;
-    unsigned grid = blocks_for(c->A.n, INFECT_TILE);
-    if (c->timing && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
+    const unsigned grid = blocks_for(c->A.n, INFECT_TILE);
+    if (c->timing == PANSIM_TIMING_KERNELS && record_event(c, c->infectEv, 2 * c->infectN)) return PANSIM_E_DEVICE;
     if (a.hazard) hipLaunchKernelGGL(k_infect<true>, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
     else hipLaunchKernelGGL(k_infect<false>, dim3(grid), dim3(INFECT_BLOCK), 0, c->stream, a);
-    if (c->timing) {
+    if (c->timing == PANSIM_TIMING_KERNELS) {
         if (record_event(c, c->infectEv, 2 * c->infectN + 1)) return PANSIM_E_DEVICE;
         c->infectN++;
+    } else if (c->pipeOpen) {  // closes the pair opened by the movement launch
+        if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
+        c->moveN++;
+        c->pipeOpen = false;
     }
     return check_launch(c, "infection");
 }
