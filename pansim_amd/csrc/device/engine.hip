@@ -506,11 +506,16 @@ static MoveArgs move_args(pansim_ctx* c, const pansim_step_params* p) {
     m.p = *p;
     m.seed = c->seed;
     m.replica = c->replica;
-    m.stepOfDay = p->timestamp % (1440u / c->dt);
-    m.curfewStep = (p->enable_curfew && p->curfew_begin == m.stepOfDay) ? 1u : 0u;
     m.nEvents = c->nEvents;
     m.counters = c->moveCounters;
     m.pendPar = c->pendPar;
+    const uint32_t dt = p->time_step;
+    m.spd = 1440u / dt;
+    m.qLen = p->quarantine_length * 24u * 60u / dt;
+    m.hourSteps = 60u / dt;
+    m.halfHourSteps = 30u / dt;
+    m.dtMagic = ((1ull << 32) + dt - 1u) / dt;
+    move_set_step(m, p->timestamp, p->timestamp % m.spd);
     return m;
 }
 

@@ -173,10 +173,27 @@ __device__ __forceinline__ CsrReg load_csr(const Agents& A, const AgentRec& r) {
     return c;
 }
 
-__device__ __forceinline__ uint32_t csr_type(const CsrReg& c, uint32_t k) {
-    // selects, not an indexed array, so that a run-time k never spills the words to memory
-    const uint32_t w = k < 8u ? (k < 4u ? c.t[0] : c.t[1]) : (k < 12u ? c.t[2] : c.t[3]);
-    return (w >> (8u * (k & 3u))) & 0xFFu;
+// Entries of the register list whose type is ty, one bit per entry (bit k =
+// entry k).  Four type bytes per word are compared at once: byte b of x is
+// zero exactly when entry b matches, and t below has bit 7 of that byte set
+// exactly then (the 7-bit sums cannot carry between bytes); the four bit-7
+// flags are gathered into bits 28..31 by one multiplication.
+__device__ __forceinline__ uint32_t csr_match(const CsrReg& c, uint32_t ty) {
+    const uint32_t rep = ty * 0x01010101u;
+    uint32_t mask = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t x = c.t[j] ^ rep;
+        const uint32_t t = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+        mask |= (((t >> 7) * 0x10204080u) >> 28) << (4u * j);
+    }
+    return mask & ((1u << c.n) - 1u);  // entries past the list (n <= 16 here)
+}
+
+// Entry index of the r-th (0-based) set bit of mask; r < popcount(mask).
+__device__ __forceinline__ uint32_t nth_bit(uint32_t mask, uint32_t r) {
+    for (uint32_t k = 0; k < r; ++k) mask &= mask - 1u;
+    return (uint32_t)__builtin_ctz(mask);
 }
 
 // findActualLocationForType over the register copy of the types; lists longer
@@ -185,21 +202,12 @@ __device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, con
                                              const CsrReg& c, uint32_t ty, int retry, Stream* rs) {
     if (ty == lt.home) return c.home;
     if (c.n > CSR_REG_MAX) return find_loc(A, L, lt, Csr{c.b, c.b + c.n, c.home}, ty, retry, rs);
-    if (ty == lt.school || ty == lt.work) {
-#pragma unroll
-        for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
-            if (k < c.n && csr_type(c, k) == ty) return A.possLoc[c.b + k];
-        return NOLOC;
-    }
-    uint32_t counter = 0, pos = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
-        if (k < c.n && csr_type(c, k) == ty) {
-            if (counter == 0) pos = k;
-            ++counter;
-        }
-    if (counter == 0) return NOLOC;
-    if (counter == 1) return A.possLoc[c.b + pos];
+    const uint32_t mask = ty <= 0xFFu ? csr_match(c, ty) : 0u;  // list types are bytes
+    if (mask == 0u) return NOLOC;
+    // school and work: the first entry of the type, no draw
+    if (ty == lt.school || ty == lt.work || (mask & (mask - 1u)) == 0u)
+        return A.possLoc[c.b + (uint32_t)__builtin_ctz(mask)];
+    const uint32_t counter = (uint32_t)__builtin_popcount(mask);
     uint32_t counter2 = counter, randIdx = counter;
     int redraws = 0;
     for (;;) {
@@ -211,14 +219,7 @@ __device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, con
             randIdx = rs->uint_below(counter);
             ++redraws;
         }
-        uint32_t sel = 0, seen = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < CSR_REG_MAX; ++k)
-            if (k < c.n && csr_type(c, k) == ty) {
-                if (seen == randIdx) sel = k;
-                ++seen;
-            }
-        const uint32_t l = A.possLoc[c.b + sel];
+        const uint32_t l = A.possLoc[c.b + nth_bit(mask, randIdx)];
         if (retry && L.rec[l].open == 0) {  // closed: draw another one, once
             retry = 0;
             counter2 = randIdx;
@@ -247,7 +248,32 @@ struct MoveArgs {
     uint32_t nEvents;
     uint32_t pendPar;  // parity of the pending-write counter used this step
     unsigned long long* counters;  // MOVE_CTR_SLOTS pairs (evaluated, relocated)
+    // step constants (move_set_step): minutes into the day, steps left until
+    // midnight, first step of the day
+    uint32_t minutes, stepsUntilMidnight, dayStart;
+    // run constants (set by the host): steps per day, quarantine length in
+    // steps, steps per hour and per half hour, ceil(2^32 / time step)
+    uint32_t spd, qLen, hourSteps, halfHourSteps;
+    uint64_t dtMagic;
 };
+
+// The step-dependent members of m for timestamp ts (stepOfDay = ts % steps
+// per day).
+__host__ __device__ inline void move_set_step(MoveArgs& m, uint32_t ts, uint32_t stepOfDay) {
+    m.p.timestamp = ts;
+    m.stepOfDay = stepOfDay;
+    m.curfewStep = (m.p.enable_curfew && m.p.curfew_begin == stepOfDay) ? 1u : 0u;
+    m.minutes = stepOfDay * m.p.time_step;
+    m.stepsUntilMidnight = (1440u - m.minutes) / m.p.time_step;
+    m.dayStart = ts - stepOfDay;
+}
+
+// x / time step for x < 2^16 (minute counts of the day's clock): with
+// magic = ceil(2^32 / d), floor(x * magic / 2^32) is exact for d <= 1440 and
+// x < 2^32 / d (checked exhaustively for all such d and x < 2^16).
+__device__ __forceinline__ uint32_t div_dt(const MoveArgs& m, uint32_t x) {
+    return (uint32_t)(((uint64_t)x * m.dtMagic) >> 32);
+}
 
 constexpr uint32_t MOVE_BLOCK = 256;
 constexpr uint32_t MOVE_ITEMS = 16;
@@ -301,15 +327,14 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     const Tables& T = m.T;
     const pansim_location_types& lt = T.lt;
     const uint32_t ts = m.p.timestamp;
-    const uint32_t dt = m.p.time_step;
-    const unsigned minutes = (ts % (1440u / dt)) * dt;
+    const unsigned minutes = m.minutes;
     const TD simTime = td_from_min(minutes);
-    const uint32_t stepsUntilMidnight = (1440u - minutes) / dt;
-    const uint32_t qLen = m.p.quarantine_length * 24u * 60u / dt;
+    const uint32_t stepsUntilMidnight = m.stepsUntilMidnight;
+    const uint32_t qLen = m.qLen;
     const uint32_t agentType = r.type;
     const uint32_t oldLoc = A.location[i];
     uint32_t loc = oldLoc;
-    const uint32_t dayStart = ts - minutes / dt;
+    const uint32_t dayStart = m.dayStart;
     const uint16_t wake0 = A.wake[i];
     uint32_t wake = wake0 == WAKE_NONE ? NEVER : dayStart + wake0;
     Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
@@ -330,7 +355,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         A.quarTs[i] = ts;
         const uint32_t prev = r.quarUntil;
         r.quarUntil = until;
-        A.daysInQ[i] += (until - ts) / (24u * 60u / dt);
+        A.daysInQ[i] += (until - ts) / m.spd;
         if (r.flags & F_DIAGNOSED) {
             qLoc = true;
             qLocUntil = until;
@@ -340,7 +365,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
 
     bool done = false;
     // curfew (movementPolicies.h:267-326)
-    if (m.p.enable_curfew && m.p.curfew_begin == minutes / dt) {
+    if (m.p.enable_curfew && m.p.curfew_begin == m.stepOfDay) {
         bool deadOrHosp = (w == 4 || w == 3);
         bool hospNonCovid = r.hospTs <= ts && r.hospUntil > ts;
         if (!deadOrHosp && !hospNonCovid) {
@@ -376,7 +401,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             if (!(r.flags & F_DIAGNOSED) && T.stInf[st] * T.varMul[r.variant] > 0.0f) {
                 r.flags |= F_DIAGNOSED;
                 uint32_t d = ts;
-                if (stepsUntilMidnight == 24u * 60u / dt) d++;
+                if (stepsUntilMidnight == m.spd) d++;
                 r.diagTs = d;
                 quarantine(ts + qLen);
             }
@@ -401,7 +426,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             return t == lt.home || t == lt.school || t == lt.classroom || t == lt.work;
         };
         auto stayedHomeCheck = [&]() {
-            if (agentHome != NOLOC && loc != agentHome && sum > 60u / dt) r.flags &= (uint8_t)~F_STAYED_HOME;
+            if (agentHome != NOLOC && loc != agentHome && sum > m.hourSteps) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
         if (m.p.quarantine_policy > 1) {
             const LocRec lr = lrec(L, loc);
@@ -433,7 +458,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         }
         // schedule window (agentType.cpp:13-15), precomputed per step of the day
         const uint32_t off = 7u * w + m.p.day_of_week + agentType * 28u;
-        const int2 win = T.evWin[(size_t)(minutes / dt) * T.nOff + off];
+        const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff + off];
         const int ab = win.x, ae = win.y;
         uint32_t newType = NOLOC;
         if (ab == -1 && ae == -1) {  // Case 1
@@ -500,7 +525,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 sum = stepsUntilMidnight;
             } else if (ae == -1) {
                 TD arrive = td_plus(simTime, dur);
-                sum = arrive.h >= 24 ? stepsUntilMidnight : td_min(dur) / dt;
+                sum = arrive.h >= 24 ? stepsUntilMidnight : div_dt(m, td_min(dur));
             } else {
                 // ae can equal the slot end: the reference reads the next slot's first
                 // event, or a default event past the end of the array
@@ -512,16 +537,16 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                     ne.start_h = ne.start_m = ne.end_h = ne.end_m = ne.dur_h = ne.dur_m = 0;
                 }
                 TD arrive = td_plus(simTime, dur);
-                if (td_lt(arrive, ev_start(ne))) sum = td_min(dur) / dt;
-                else if (td_lt(ev_end(ne), arrive)) sum = td_min(td_minus(ev_end(ne), simTime)) / dt - 1u;
+                if (td_lt(arrive, ev_start(ne))) sum = div_dt(m, td_min(dur));
+                else if (td_lt(ev_end(ne), arrive)) sum = div_dt(m, td_min(td_minus(ev_end(ne), simTime))) - 1u;
                 else {
-                    uint32_t stp = td_min(td_minus(ev_end(ne), arrive)) / dt;
+                    uint32_t stp = div_dt(m, td_min(td_minus(ev_end(ne), arrive)));
                     uint32_t extra = rs.uint_below(stp);
-                    sum = td_min(dur) / dt + extra;
+                    sum = div_dt(m, td_min(dur)) + extra;
                 }
             }
             if (agentType + 1 == 7 && m.p.enable_curfew &&
-                (m.p.curfew_begin <= minutes / dt || m.p.curfew_end > minutes / dt)) {
+                (m.p.curfew_begin <= m.stepOfDay || m.p.curfew_end > m.stepOfDay)) {
                 const LocRec lr = lrec(L, newLoc);
                 bool workButClosed = newType == lt.work && (lr.open == 0 || lr.closedUntil > ts);
                 if (workButClosed || (newType != lt.work && newType != lt.home)) loc = agentHome;
@@ -529,11 +554,11 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         }
         if (ab == -1 && ae != -1) {  // Case 3
             const pansim_event& ne = T.ev[ae];
-            uint32_t lenSteps = td_min(td_minus(ev_end(ne), ev_start(ne))) / dt;
+            uint32_t lenSteps = div_dt(m, td_min(td_minus(ev_end(ne), ev_start(ne))));
             uint32_t delay = rs.uint_below(lenSteps);
-            sum = td_min(td_minus(ev_start(ne), simTime)) / dt + delay;
-            if (sum < 30u / dt) {
-            } else if (sum < 60u / dt) {
+            sum = div_dt(m, td_min(td_minus(ev_start(ne), simTime))) + delay;
+            if (sum < m.halfHourSteps) {
+            } else if (sum < m.hourSteps) {
                 newType = lt.public_space;
                 loc = find_loc(A, L, lt, csr, lt.public_space, 0, &rs);
             } else {
@@ -565,7 +590,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     if (r.diagTs != r0.diagTs) A.rec[i].diagTs = r.diagTs;
     if (r.quarUntil != r0.quarUntil) A.rec[i].quarUntil = r.quarUntil;
     // a wake step beyond today is never reached: the midnight planning resets it
-    const uint16_t wake16 = (wake == NEVER || wake - dayStart >= 1440u / dt) ? WAKE_NONE : (uint16_t)(wake - dayStart);
+    const uint16_t wake16 = (wake == NEVER || wake - dayStart >= m.spd) ? WAKE_NONE : (uint16_t)(wake - dayStart);
     if (wake16 != wake0) A.wake[i] = wake16;
     // a location type the agent has no entry for leaves it where it is
     if (loc == NOLOC) loc = oldLoc;
@@ -829,10 +854,8 @@ struct BatchArgs {
 __global__ void __launch_bounds__(MOVE_BLOCK) k_move_batch(BatchArgs b) {
     const uint32_t r = b.moveRep[blockIdx.x];
     MoveArgs m = b.move[r];
-    m.p.timestamp = b.ts;
-    m.stepOfDay = b.stepOfDay;
+    move_set_step(m, b.ts, b.stepOfDay);
     m.pendPar = b.pendPar;
-    m.curfewStep = (m.p.enable_curfew && m.p.curfew_begin == b.stepOfDay) ? 1u : 0u;
     move_tile(m, blockIdx.x - b.moveFirst[r]);
 }
 
