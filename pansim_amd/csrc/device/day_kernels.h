@@ -169,7 +169,7 @@ __global__ void k_flag_locations(TestArgs a) {
         uint32_t q = a.L.rec[l].quarUntil;
         return q == 0 || (q != 0 && (q - ql >= a.ts - day || q < a.ts));
     };
-    const CsrReg csr = load_csr(a.A, a.A.rec[i]);
+    const CsrReg csr = load_csr(a.A, i, a.A.rec[i]);
     uint32_t home = csr.home;
     if (home != NOLOC) a.L.rec[home].flag = 1;
     uint32_t work = find_loc(a.A, a.L, lt, csr, lt.work, 0, nullptr);
@@ -199,7 +199,7 @@ __global__ void k_do_testing(TestArgs a) {
         uint8_t st = A.rec[i].state;
         if (!skip && !(a.T.stWB[st] == 4 || (fl & F_DIAGNOSED))) {
             Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
-            const CsrReg csr = load_csr(A, A.rec[i]);
+            const CsrReg csr = load_csr(A, i, A.rec[i]);
             uint32_t home = csr.home;
             bool homeFlag = home != NOLOC ? a.L.rec[home].flag != 0 : false;
             uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
@@ -802,7 +802,7 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
     if (schoolwork && !(fl & F_QUARANTINED) && !(r.diagTs > 0 && !a.T.stInfected[r.state])) {
         const pansim_location_types& lt = a.T.lt;
         Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
-        const CsrReg csr = load_csr(A, r);
+        const CsrReg csr = load_csr(A, i, r);
         uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
         uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
         uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
@@ -855,7 +855,7 @@ __global__ void k_plan_underage(PlanArgs a) {
     if (A.rec[i].flags & F_QUARANTINED) flag = true;
     else {
         Stream rs(a.seed, i, a.ts, PH_PLAN_UNDERAGE, a.replica);
-        const CsrReg csr = load_csr(A, A.rec[i]);
+        const CsrReg csr = load_csr(A, i, A.rec[i]);
         uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
         uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
         bool tooOld = A.rec[i].age >= a.schoolAge;

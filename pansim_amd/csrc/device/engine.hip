@@ -373,13 +373,17 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(dalloc(c, &A.worstState, N));
     TRY(upload(c, (uint32_t**)&A.possLoc, a->possible_locations, d->n_csr));
     {
-        // padded by 16 zero bytes: the movement kernel reads the types with aligned 4-byte loads
-        uint8_t* pt = nullptr;
-        TRY(dalloc(c, &pt, (size_t)d->n_csr + 16));
-        HIP_OR_FAIL(c, hipMemsetAsync(pt, 0, (size_t)d->n_csr + 16, c->stream));
-        if (d->n_csr)
-            HIP_OR_FAIL(c, hipMemcpyAsync(pt, a->possible_types, d->n_csr, hipMemcpyHostToDevice, c->stream));
-        A.possType = pt;
+        TRY(upload(c, (uint8_t**)&A.possType, a->possible_types, d->n_csr));
+        // the types of every list of at most 16 entries as one 16-byte record per agent
+        std::vector<uint32_t> t16((size_t)N * 4, 0u);
+        for (uint32_t k = 0; k < N; ++k) {
+            const uint32_t b = a->loc_offset[k], e = a->loc_offset[k + 1];
+            if (e - b > 16u) continue;
+            for (uint32_t j = b; j < e; ++j)
+                t16[(size_t)k * 4 + (j - b) / 4] |= (uint32_t)a->possible_types[j] << (8u * ((j - b) % 4));
+        }
+        TRY(upload(c, (uint32_t**)&A.types16, t16.data(), (size_t)N * 4));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
     // ---- scratch
     c->S.pendCap = d->n_locations + 16u;

@@ -111,6 +111,7 @@ struct Agents {
     // possible-location lists
     const uint32_t* possLoc;
     const uint8_t* possType;
+    const uint32_t* types16;  // [n][4]: the type bytes of each list of at most 16 entries, zero padded
 };
 
 struct Locs {

@@ -154,21 +154,21 @@ struct CsrReg {
     uint32_t t[4];
 };
 
-__device__ __forceinline__ CsrReg load_csr(const Agents& A, const AgentRec& r) {
+// Agent i's list (record r): the types of a short list come from its
+// 16-byte record in A.types16, one aligned load.
+__device__ __forceinline__ CsrReg load_csr(const Agents& A, uint32_t i, const AgentRec& r) {
     CsrReg c;
     c.b = r.csrBegin;
     c.n = r.csrCount;
     c.home = r.home;
     c.t[0] = c.t[1] = c.t[2] = c.t[3] = 0u;
     if (c.n - 1u < CSR_REG_MAX) {  // 1..16 entries
-        const uint32_t* p = reinterpret_cast<const uint32_t*>(A.possType) + (c.b >> 2);
-        const uint32_t sh = c.b & 3u;
-        const uint32_t nw = (sh + c.n + 3u) >> 2;  // 1..5 words
-        uint32_t w[5];
-#pragma unroll
-        for (uint32_t j = 0; j < 5; ++j) w[j] = j < nw ? p[j] : 0u;
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) c.t[j] = __builtin_amdgcn_alignbyte(w[j + 1], w[j], sh);
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        const v4u t = *reinterpret_cast<const v4u*>(A.types16 + 4 * (size_t)i);
+        c.t[0] = t.x;
+        c.t[1] = t.y;
+        c.t[2] = t.z;
+        c.t[3] = t.w;
     }
     return c;
 }
@@ -321,7 +321,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     // one 32-byte load: every agent field the routine reads
     const AgentRec r0 = m.A.rec[i];
     AgentRec r = r0;
-    const CsrReg csr = load_csr(m.A, r0);
+    const CsrReg csr = load_csr(m.A, i, r0);
     const Agents& A = m.A;
     const Locs& L = m.L;
     const Tables& T = m.T;
@@ -335,11 +335,25 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     const uint32_t oldLoc = A.location[i];
     uint32_t loc = oldLoc;
     const uint32_t dayStart = m.dayStart;
-    const uint16_t wake0 = A.wake[i];
+    // a mover's wake step is this step, except on the curfew step (everyone)
+    const uint16_t wake0 = m.curfewStep ? A.wake[i] : (uint16_t)m.stepOfDay;
     uint32_t wake = wake0 == WAKE_NONE ? NEVER : dayStart + wake0;
     Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
     const uint8_t st = r.state;
     const uint8_t w = T.stWB[st];
+    // state and variant are fixed during the routine
+    const float xInf = T.stInf[st] * T.varMul[r.variant];
+    const bool infectedState = T.stInfected[st] != 0;
+    // the last location record read (movement writes no location record)
+    uint32_t lastL = NOLOC;
+    LocRec lastR;
+    auto lrec_c = [&](uint32_t l) {
+        if (l != lastL) {
+            lastR = lrec(L, l);
+            lastL = l;
+        }
+        return lastR;
+    };
 
     // quarantineAgent (movementPolicies.h:171-245) without its location
     // writes: an agent is quarantined at most once per routine, and only a
@@ -348,8 +362,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     uint32_t qLocUntil = 0, qLocPrev = 0;
     auto quarantine = [&](uint32_t until) {
         if (m.p.quarantine_policy == 0) return;
-        const bool infected = T.stInfected[r.state] != 0;
-        if (r.diagTs > 0 && !infected && m.p.quarantine_immune_active == 0) return;
+        if (r.diagTs > 0 && !infectedState && m.p.quarantine_immune_active == 0) return;
         if (!(r.flags & F_DIAGNOSED) && A.immTs[i] > 0 && m.p.quarantine_immune_active == 0) return;
         r.flags |= F_QUARANTINED;
         A.quarTs[i] = ts;
@@ -371,7 +384,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         if (!deadOrHosp && !hospNonCovid) {
             if (agentType + 1 == 7) {
                 uint32_t workplace = find_loc(A, L, lt, csr, lt.work, 0, nullptr);
-                const bool workOpen = lrec(L, workplace).open != 0;
+                const bool workOpen = lrec_c(workplace).open != 0;
                 if (loc != workplace || !workOpen) {
                     loc = csr.home;
                     if (!workOpen) {
@@ -398,7 +411,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         const uint32_t agentHome = csr.home;
         uint32_t sum = 0;
         auto diagnose = [&]() {
-            if (!(r.flags & F_DIAGNOSED) && T.stInf[st] * T.varMul[r.variant] > 0.0f) {
+            if (!(r.flags & F_DIAGNOSED) && xInf > 0.0f) {
                 r.flags |= F_DIAGNOSED;
                 uint32_t d = ts;
                 if (stepsUntilMidnight == m.spd) d++;
@@ -429,7 +442,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             if (agentHome != NOLOC && loc != agentHome && sum > m.hourSteps) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
         if (m.p.quarantine_policy > 1) {
-            const LocRec lr = lrec(L, loc);
+            const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
                 if (r.flags & F_QUARANTINED) {
@@ -495,7 +508,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             bool tooOld = (newType == lt.school || newType == lt.classroom) && r.age >= m.p.school_age_restriction;
             bool lockdown = m.p.lockdown_nonvacc_active && A.immTs[i] == 0;
             auto closedHere = [&](uint32_t l, uint32_t t) {
-                const LocRec lr = lrec(L, l);
+                const LocRec lr = lrec_c(l);
                 return (lr.open == 0 || lr.closedUntil > ts) && t != lt.work;
             };
             if (tooOld || closedHere(newLoc, newType) || lockdown) {
@@ -547,7 +560,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             }
             if (agentType + 1 == 7 && m.p.enable_curfew &&
                 (m.p.curfew_begin <= m.stepOfDay || m.p.curfew_end > m.stepOfDay)) {
-                const LocRec lr = lrec(L, newLoc);
+                const LocRec lr = lrec_c(newLoc);
                 bool workButClosed = newType == lt.work && (lr.open == 0 || lr.closedUntil > ts);
                 if (workButClosed || (newType != lt.work && newType != lt.home)) loc = agentHome;
             }
@@ -567,7 +580,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             }
         }
         if (m.p.quarantine_policy > 1) {
-            const LocRec lr = lrec(L, loc);
+            const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
                 if (r.flags & F_QUARANTINED) {
@@ -597,9 +610,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     if (loc == oldLoc) return false;
     A.location[i] = loc;
     const uint8_t var = r.variant;
-    float x = T.stInf[st] * T.varMul[var];
-    if (x != 0.0f) {
-        long long q = fixed32(x);
+    if (xInf != 0.0f) {
+        long long q = fixed32(xInf);
         atomicAdd((unsigned long long*)loc_sum(L, var, oldLoc), (unsigned long long)(-q));
         atomicAdd((unsigned long long*)loc_sum(L, var, loc), (unsigned long long)q);
         mark_visited(L, loc);
