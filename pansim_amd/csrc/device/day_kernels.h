@@ -824,21 +824,31 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
 // Per-location part: noWork reset (movementPolicies.h:1449) and, with
 // quarantinePolicy > 2, the quarantine -> closedUntil conversion
 // (movementPolicies.h:1544-1557).
+// Also recomputes the special-location bits for the day (loc_special): each
+// wave covers 64 consecutive locations (block size and grid stride are
+// multiples of 64) and writes its two bitmap words from a ballot.
 __global__ void k_plan_locs(PlanArgs a, uint32_t close) {
     uint32_t stride = gridDim.x * blockDim.x;
     const pansim_location_types& lt = a.T.lt;
+    const uint32_t dayStart = a.ts - a.ts % (1440u / a.dt);
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < a.L.n; l += stride) {
         LocRec* lr = &a.L.rec[l];
         lr->noWork = 0u;
-        if (!close) continue;
         uint32_t q = lr->quarUntil;
-        uint32_t t = lr->type;
-        if (q > a.ts && (t == lt.school || t == lt.classroom)) {
-            lr->closedUntil = q;
-            lr->quarUntil = 0;
-        } else if (q > a.ts && t == lt.work) {
-            lr->quarUntil = 0;
+        if (close) {
+            uint32_t t = lr->type;
+            if (q > a.ts && (t == lt.school || t == lt.classroom)) {
+                lr->closedUntil = q;
+                lr->quarUntil = q = 0;
+            } else if (q > a.ts && t == lt.work) {
+                lr->quarUntil = q = 0;
+            }
         }
+        const bool sp = q > dayStart || lr->closedUntil > dayStart || lr->open == 0;
+        const unsigned long long bits = __ballot(sp);
+        const uint32_t lane = threadIdx.x & 63u;
+        if (lane == 0u) a.L.special[l >> 5] = (uint32_t)bits;
+        if (lane == 32u) a.L.special[l >> 5] = (uint32_t)(bits >> 32);
     }
 }
 
@@ -895,7 +905,10 @@ __global__ void k_set_open(Locs L, const uint32_t* types, uint32_t n, uint8_t op
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
         if (L.rec[l].essential == 1) continue;
         for (uint32_t k = 0; k < n; ++k)
-            if (L.rec[l].type == types[k]) L.rec[l].open = open;
+            if (L.rec[l].type == types[k]) {
+                L.rec[l].open = open;
+                if (!open) mark_special(L, l);
+            }
     }
 }
 

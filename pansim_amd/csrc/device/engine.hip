@@ -299,6 +299,9 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));
     TRY(dalloc(c, &L.visited, ((size_t)NL + 31) / 32));
     HIP_OR_FAIL(c, hipMemsetAsync(L.visited, 0, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
+    // every location special until the first midnight planning computes the bits
+    TRY(dalloc(c, &L.special, ((size_t)NL + 31) / 32));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.special, 0xFF, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &L.pendKey, NL));
     {
         std::vector<float> ones(HAZARD_TYPES, 1.0f);
@@ -943,7 +946,6 @@ int pansim_upload_schedules(pansim_ctx* c, const uint32_t* off, uint32_t nOff, c
     if (nEv) HIP_OR_FAIL(c, hipMemcpy(c->evDev, ev, nEv * sizeof(pansim_event), hipMemcpyHostToDevice));
     c->nEvents = nEv;
     return compute_windows(c);
-    return 0;
 }
 
 int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_t n) {

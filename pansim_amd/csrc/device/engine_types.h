@@ -121,12 +121,23 @@ struct Locs {
     LocDyn* dyn;
     long long* sumX;                // [V-1][n]: sums of variants 1..V-1
     uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
+    uint32_t* special;              // [(n + 31) / 32] bit l clear: l is open, not closed and not quarantined all day
     unsigned long long* pendKey;    // [n]
     const uint32_t* schools;
     const uint32_t* classrooms;
     const uint32_t* classOff;
     uint32_t nSchools;
 };
+
+// Whether movement must read location l's record to know if it is closed
+// or quarantined: a clear bit means open, closedUntil and quarUntil at most
+// the day's first step.  The midnight planning recomputes the bits; every
+// later write that can close or quarantine a location sets its bit.  A
+// missing location (NOLOC) always reads its (closed) record.
+__device__ __forceinline__ bool loc_special(const Locs& L, uint32_t l) {
+    return l >= L.n || ((L.special[l >> 5] >> (l & 31u)) & 1u) != 0u;
+}
+__device__ __forceinline__ void mark_special(const Locs& L, uint32_t l) { atomicOr(&L.special[l >> 5], 1u << (l & 31u)); }
 
 // Marks location l as visited by an infectious agent (bits are only cleared
 // by the full rebuild at midnight, so a set bit may have a zero sum again).

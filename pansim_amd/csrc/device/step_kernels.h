@@ -131,7 +131,7 @@ __device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_locati
             for (uint32_t i = pos; i < e; ++i) {
                 if (A.possType[i] == ty) {
                     if (counter2 == randIdx) {
-                        if (retry && L.rec[A.possLoc[i]].open == 0) {
+                        if (retry && loc_special(L, A.possLoc[i]) && L.rec[A.possLoc[i]].open == 0) {
                             retry = 0;
                             break;
                         }
@@ -220,7 +220,7 @@ __device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, con
             ++redraws;
         }
         const uint32_t l = A.possLoc[c.b + nth_bit(mask, randIdx)];
-        if (retry && L.rec[l].open == 0) {  // closed: draw another one, once
+        if (retry && loc_special(L, l) && L.rec[l].open == 0) {  // closed: draw another one, once
             retry = 0;
             counter2 = randIdx;
             continue;
@@ -384,7 +384,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         if (!deadOrHosp && !hospNonCovid) {
             if (agentType + 1 == 7) {
                 uint32_t workplace = find_loc(A, L, lt, csr, lt.work, 0, nullptr);
-                const bool workOpen = lrec_c(workplace).open != 0;
+                const bool workOpen = !loc_special(L, workplace) || lrec_c(workplace).open != 0;
                 if (loc != workplace || !workOpen) {
                     loc = csr.home;
                     if (!workOpen) {
@@ -441,7 +441,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         auto stayedHomeCheck = [&]() {
             if (agentHome != NOLOC && loc != agentHome && sum > m.hourSteps) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
-        if (m.p.quarantine_policy > 1) {
+        // (a location that is not special is not quarantined today)
+        if (m.p.quarantine_policy > 1 && loc_special(L, loc)) {
             const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
@@ -508,8 +509,9 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             bool tooOld = (newType == lt.school || newType == lt.classroom) && r.age >= m.p.school_age_restriction;
             bool lockdown = m.p.lockdown_nonvacc_active && A.immTs[i] == 0;
             auto closedHere = [&](uint32_t l, uint32_t t) {
+                if (t == lt.work || !loc_special(L, l)) return false;
                 const LocRec lr = lrec_c(l);
-                return (lr.open == 0 || lr.closedUntil > ts) && t != lt.work;
+                return lr.open == 0 || lr.closedUntil > ts;
             };
             if (tooOld || closedHere(newLoc, newType) || lockdown) {
                 if (nPot > 1) {
@@ -531,7 +533,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 }
             } else if (newType == lt.school) {
                 uint32_t cls = find_loc(A, L, lt, csr, lt.classroom, 0, &rs);
-                if (cls != NOLOC && L.rec[cls].closedUntil > ts) newLoc = csr.home;
+                if (cls != NOLOC && loc_special(L, cls) && L.rec[cls].closedUntil > ts) newLoc = csr.home;
             }
             loc = newLoc;
             if (dur.h > 24) {
@@ -560,8 +562,11 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             }
             if (agentType + 1 == 7 && m.p.enable_curfew &&
                 (m.p.curfew_begin <= m.stepOfDay || m.p.curfew_end > m.stepOfDay)) {
-                const LocRec lr = lrec_c(newLoc);
-                bool workButClosed = newType == lt.work && (lr.open == 0 || lr.closedUntil > ts);
+                bool workButClosed = false;
+                if (newType == lt.work && loc_special(L, newLoc)) {
+                    const LocRec lr = lrec_c(newLoc);
+                    workButClosed = lr.open == 0 || lr.closedUntil > ts;
+                }
                 if (workButClosed || (newType != lt.work && newType != lt.home)) loc = agentHome;
             }
         }
@@ -579,7 +584,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 loc = csr.home;
             }
         }
-        if (m.p.quarantine_policy > 1) {
+        // (a location that is not special is not quarantined today)
+        if (m.p.quarantine_policy > 1 && loc_special(L, loc)) {
             const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
@@ -691,7 +697,10 @@ __device__ __forceinline__ void resolve_pending(const Locs& L, const StepScratch
     for (uint32_t j = blk * blockDim.x + threadIdx.x; j < pc; j += nBlk * blockDim.x) {
         uint32_t loc = S.pendList[j];
         unsigned long long key = atomicExch(&L.pendKey[loc], 0ull);
-        if (key != 0ull) L.rec[loc].quarUntil = (uint32_t)key;
+        if (key != 0ull) {
+            L.rec[loc].quarUntil = (uint32_t)key;
+            mark_special(L, loc);
+        }
     }
 }
 
