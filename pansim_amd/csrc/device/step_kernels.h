@@ -810,37 +810,33 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
         loc[0] = l0.x; loc[1] = l0.y; loc[2] = l0.z; loc[3] = l0.w;
         loc[4] = l1.x; loc[5] = l1.y; loc[6] = l1.z; loc[7] = l1.w;
     }
-    for (uint32_t v = 0; v < a.T.nVar; ++v) {
-        if (!((a.mask >> v) & 1u)) continue;
-        // susceptibility is re-read per variant: an infection by an earlier
-        // variant in this launch has already rewritten it (padding reads 0)
-        float s[INFECT_ITEMS];
-        {
-            const v4f* sv4 = reinterpret_cast<const v4f*>(a.A.susc + (size_t)v * a.A.suscStride + base);
-            const v4f s0 = sv4[0], s1 = sv4[1];
-            s[0] = s0.x; s[1] = s0.y; s[2] = s0.z; s[3] = s0.w;
-            s[4] = s1.x; s[5] = s1.y; s[6] = s1.z; s[7] = s1.w;
-        }
-        // the visited bit of each susceptible agent's location (an L2-resident
-        // bitmap), then the location's sum of this variant where it is set
-        uint32_t vis[INFECT_ITEMS];
+    // the visited bit of each agent's location (an L2-resident bitmap): an
+    // agent at a location without an infectious visitor today is done, for
+    // every variant, without reading its susceptibility
+    uint32_t cand = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) vis[k] = s[k] > 0.0f ? a.L.visited[loc[k] >> 5] : 0u;
+    for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
+        if ((a.L.visited[loc[k] >> 5] >> (loc[k] & 31u)) & 1u) cand |= 1u << k;
+    for (uint32_t v = 0; cand != 0u && v < a.T.nVar; ++v) {
+        if (!((a.mask >> v) & 1u)) continue;
+        // the location's sum of this variant, then the susceptibility, which
+        // is re-read per variant: an infection by an earlier variant in this
+        // launch has already rewritten it
         const long long* sv = v == 0 ? &a.L.dyn[0].sum0 : a.L.sumX + (size_t)(v - 1) * a.L.n;
         const uint32_t pitch = v == 0 ? (uint32_t)(sizeof(LocDyn) / sizeof(long long)) : 1u;
+        const float* su = a.A.susc + (size_t)v * a.A.suscStride + base;
         uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
-            if (((vis[k] >> (loc[k] & 31u)) & 1u) && sv[(size_t)loc[k] * pitch] != 0) hits |= 1u << k;
+            if (((cand >> k) & 1u) && sv[(size_t)loc[k] * pitch] != 0 && su[k] > 0.0f) hits |= 1u << k;
         // the rare agents at a location with infectious visitors: one at a time
         while (hits) {
             const uint32_t k = (uint32_t)__builtin_ctz(hits);
             hits &= hits - 1u;
             const uint32_t i = base + k;
-            const uint32_t l = a.A.location[i];
+            const uint32_t l = loc[k];
             const LocDyn d = a.L.dyn[l];
-            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf,
-                                 a.A.susc[(size_t)v * a.A.suscStride + i], sh);
+            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, su[k], sh);
         }
     }
     if (HAZARD) {
