@@ -733,12 +733,12 @@ struct InfectArgs {
 // Infection of all active variants.  LocationsList::infectAgents
 // (locationList.h:251-305) with the ratio of infectionPolicies.h:523-546
 // evaluated from the location's exact sums.  Each thread takes INFECT_ITEMS
-// consecutive agents and issues their independent loads together (16-byte
-// vectors), so that a wave keeps several memory round trips in flight.
+// agents 64 apart and issues their independent loads together, so that a
+// wave keeps several memory round trips in flight.
 constexpr uint32_t INFECT_BLOCK = 256;
-constexpr uint32_t INFECT_ITEMS = 8;  // two 16-byte vectors of agents per thread
+constexpr uint32_t INFECT_ITEMS = 8;  // agents per thread
 constexpr uint32_t INFECT_TILE = INFECT_BLOCK * INFECT_ITEMS;
-static_assert(INFECT_ITEMS == 8, "the infection pass loads two 4-agent vectors per thread");
+static_assert(INFECT_BLOCK % 64 == 0, "the infection pass maps 64-agent runs to waves");
 
 // The expected-infection hazard of one susceptible agent at a location with
 // infectious visitors, s * (1 - exp(-x)) with x = k * density * inf * m_type *
@@ -798,18 +798,16 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
         for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
         __syncthreads();
     }
-    // thread t takes agents base + 8t .. 8t + 7 with 16-byte loads (the
-    // location and susceptibility arrays are padded to whole tiles)
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    const uint32_t base = tile * INFECT_TILE + threadIdx.x * INFECT_ITEMS;
+    // each wave takes 64 * INFECT_ITEMS consecutive agents, item k of lane
+    // l being agent base + 64k: every load instruction covers 64 consecutive
+    // agents, so household members (consecutive agents at one location, and
+    // neighbouring households' homes) share cache lines of the bitmap below
+    // (the location and susceptibility arrays are padded to whole tiles)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t base = tile * INFECT_TILE + (threadIdx.x >> 6) * (64u * INFECT_ITEMS) + lane;
     uint32_t loc[INFECT_ITEMS];
-    {
-        const v4u* lv = reinterpret_cast<const v4u*>(a.A.location + base);
-        const v4u l0 = __builtin_nontemporal_load(&lv[0]), l1 = __builtin_nontemporal_load(&lv[1]);
-        loc[0] = l0.x; loc[1] = l0.y; loc[2] = l0.z; loc[3] = l0.w;
-        loc[4] = l1.x; loc[5] = l1.y; loc[6] = l1.z; loc[7] = l1.w;
-    }
+#pragma unroll
+    for (uint32_t k = 0; k < INFECT_ITEMS; ++k) loc[k] = __builtin_nontemporal_load(&a.A.location[base + 64u * k]);
     // the visited bit of each agent's location (an L2-resident bitmap): an
     // agent at a location without an infectious visitor today is done, for
     // every variant, without reading its susceptibility
@@ -828,15 +826,15 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
         uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
-            if (((cand >> k) & 1u) && sv[(size_t)loc[k] * pitch] != 0 && su[k] > 0.0f) hits |= 1u << k;
+            if (((cand >> k) & 1u) && sv[(size_t)loc[k] * pitch] != 0 && su[64u * k] > 0.0f) hits |= 1u << k;
         // the rare agents at a location with infectious visitors: one at a time
         while (hits) {
             const uint32_t k = (uint32_t)__builtin_ctz(hits);
             hits &= hits - 1u;
-            const uint32_t i = base + k;
+            const uint32_t i = base + 64u * k;
             const uint32_t l = loc[k];
             const LocDyn d = a.L.dyn[l];
-            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, su[k], sh);
+            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, su[64u * k], sh);
         }
     }
     if (HAZARD) {
