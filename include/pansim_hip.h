@@ -277,6 +277,20 @@ int pansim_download_field(pansim_ctx* ctx, const char* name, uint32_t timestamp,
 /* Per-location infectiousness sum of one variant, 2^-32 fixed point. */
 int pansim_download_location_sums(pansim_ctx* ctx, uint32_t variant, int64_t* out, size_t n);
 
+/* Infectious-list dump (BasicInfection::dumpLocationInfectiousList,
+ * infectionPolicies.h:262-441, DumpStore :27-85).  While enabled, every
+ * infection launch records the agents it infects.  pansim_infection_dump_collect
+ * then lists the agents infected since the previous collection with their
+ * locations, and every agent of non-zero infectiousness at those locations
+ * with its location and infectiousness (both lists unordered).  It stores the
+ * counts in *n_new / *n_inf; with all five arrays null it only counts and
+ * keeps the record, otherwise it fills the arrays (capacities cap_new /
+ * cap_inf, PANSIM_E_ARG when too small) and clears the record. */
+int pansim_infection_dump_enable(pansim_ctx* ctx, uint32_t on);
+int pansim_infection_dump_collect(pansim_ctx* ctx, uint32_t* new_agent, uint32_t* new_loc, size_t cap_new, size_t* n_new,
+                                  uint32_t* inf_agent, uint32_t* inf_loc, float* inf_value, size_t cap_inf,
+                                  size_t* n_inf);
+
 /* Device timing with HIP events on the context stream.
  * PANSIM_TIMING_KERNELS: every pansim_movement_step and pansim_infection_step
  *   is bracketed by its own event pair; pansim_timing_summary returns the

@@ -36,8 +36,14 @@ void pansim_sim_destroy(pansim_sim* sim);
 /* n iterations of runSimulation's loop (midnight block when due, movement,
  * closure step, infection, ++time). */
 int pansim_sim_steps(pansim_sim* sim, uint32_t n);
-/* Simulation::runSimulation: all weeks*7*stepsPerDay steps. */
+/* Simulation::runSimulation: all weeks*7*stepsPerDay steps, then the output
+ * files (--outAgentStat, --dumpLocationInfectiousList) and the --diags
+ * summary. */
 int pansim_sim_run(pansim_sim* sim);
+/* Simulation::finalize (simulation.h:958-961), which the reference's
+ * SimulatorInterface calls at its destruction (matlab/mexPanSim.cpp:70-73):
+ * the same output files, for a simulation driven by steps or runForDay. */
+int pansim_sim_finalize(pansim_sim* sim);
 /* Simulation::runForDay with processFlags strings (TPdef, PL0, CF2000-0500,
  * SO12, QU2, MA0.8, ...).  Writes the day's statistics vector. */
 int pansim_sim_run_for_day(pansim_sim* sim, int nflags, const char* const* flags, uint32_t* out, size_t cap, size_t* len);

@@ -41,8 +41,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <functional>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -317,6 +319,16 @@ public:
     void midnight();
     void step();  // one iteration of runSimulation's loop
     void processFlags(const std::vector<std::string>& flags);
+    // output files (agentsList.h:280-327, infectionPolicies.h:27-114)
+    void finalize();
+    void writeAgentStatJSON(const std::string& path) const;
+    std::string agentStatDiags() const;
+    std::string diagsOut;  // what --diags printed at finalize
+    // DumpStore (infectionPolicies.h:27-85)
+    std::vector<unsigned> dTs, dVar, dLocIds, dLocOff{0}, dNew, dNewOff{0}, dInf, dInfOff{0};
+    std::vector<float> dInfness;
+    void dumpInsert(unsigned variant, const std::vector<unsigned>& newlyInfected);
+    void writeDump(const std::string& path) const;
     std::vector<unsigned> runForDay(const std::vector<std::string>& flags);
 
     // helpers
@@ -966,6 +978,7 @@ void Sim::infectionsAtLocations(unsigned variant) {
             }
         }
     }
+    std::vector<unsigned> newlyInfected;  // ascending agent index
     for (unsigned i = 0; i < N; ++i) {
         PPState& p = pp[i];
         if (p.susceptible[variant] > 0.0f) {
@@ -973,6 +986,7 @@ void Sim::infectionsAtLocations(unsigned variant) {
             if (rl == 0.0f) continue;  // u < 0 never holds; the draw is counter-addressed
             Rng rng(seed, i, timestamp, PHASE_INFECT + variant);
             if (rng.unit() < (double)(rl * p.susceptible[variant])) {
+                newlyInfected.push_back(i);
                 gotInfected(p, (uint8_t)variant);
                 Stats& st = stats[i];
                 st.infectedTimestamp = timestamp;
@@ -984,6 +998,7 @@ void Sim::infectionsAtLocations(unsigned variant) {
             }
         }
     }
+    if (!sc.opt.dumpLocationInfectiousList.empty()) dumpInsert(variant, newlyInfected);
 }
 
 // movementPolicies.h:1428-1622
@@ -1996,6 +2011,144 @@ std::vector<unsigned> Sim::runForDay(const std::vector<std::string>& flags) {
     return st;
 }
 
+// dumpLocationInfectiousList (infectionPolicies.h:262-441) and
+// DumpStore::insert (:46-59): the locations where this pass infected
+// someone (ascending), the infected agents by location, and at each of those
+// locations the agents with non-zero infectiousness (ascending index, the
+// order of the stable sort by location) with their infectiousness.
+void Sim::dumpInsert(unsigned variant, const std::vector<unsigned>& newlyInfected) {
+    if (newlyInfected.empty()) return;
+    std::vector<unsigned> locs;
+    for (unsigned i : newlyInfected) locs.push_back(location[i]);
+    std::sort(locs.begin(), locs.end());
+    locs.erase(std::unique(locs.begin(), locs.end()), locs.end());
+    std::vector<int> slot(L, -1);
+    for (size_t q = 0; q < locs.size(); ++q) slot[locs[q]] = (int)q;
+    std::vector<std::vector<unsigned>> newBy(locs.size()), infBy(locs.size());
+    for (unsigned i : newlyInfected) newBy[slot[location[i]]].push_back(i);
+    for (unsigned i = 0; i < N; ++i)
+        if (slot[location[i]] >= 0 && pp[i].infectious != 0.0f) infBy[slot[location[i]]].push_back(i);
+    dTs.push_back(timestamp);
+    dVar.push_back(variant);
+    for (unsigned l : locs) dLocIds.push_back(l);
+    dLocOff.push_back((unsigned)dLocIds.size());
+    for (auto& v : newBy)
+        for (unsigned i : v) dNew.push_back(i);
+    dNewOff.push_back((unsigned)dNew.size());
+    for (auto& v : infBy) {
+        for (unsigned i : v) {
+            dInf.push_back(i);
+            dInfness.push_back(pp[i].infectious);
+        }
+        dInfOff.push_back((unsigned)dInf.size());
+    }
+}
+
+// DumpStore::write (infectionPolicies.h:60-84): the count of inserts, then
+// one line per array, every value followed by a space.
+void Sim::writeDump(const std::string& path) const {
+    std::ofstream f(path.c_str());
+    if (!f.is_open()) return;
+    f << dTs.size() << "\n";
+    auto put = [&f](const auto& v) {
+        for (const auto& x : v) f << x << " ";
+        f << "\n";
+    };
+    put(dTs);
+    put(dVar);
+    put(dLocOff);
+    put(dLocIds);
+    put(dNewOff);
+    put(dNew);
+    put(dInfOff);
+    put(dInf);
+    put(dInfness);
+}
+
+// AgentStatOutput (agentStatOutput.cpp:5-44) through rapidjson's
+// PrettyWriter (4-space indentation, JSONWriter.cpp:15-25).
+void Sim::writeAgentStatJSON(const std::string& path) const {
+    if (path.empty()) return;
+    std::ofstream f(path.c_str());
+    if (!f.is_open()) return;
+    auto quoted = [](const std::string& s) {
+        std::string o = "\"";
+        for (char ch : s) {
+            if (ch == '"' || ch == '\\') o += '\\';
+            o += ch;
+        }
+        return o + "\"";
+    };
+    std::ostringstream out;
+    out << "{\n    \"Statistics\": [";
+    for (unsigned i = 0; i < N; ++i) {
+        const Stats& e = stats[i];
+        out << (i == 0 ? "\n" : ",\n") << "        {\n";
+        const std::pair<const char*, unsigned long long> kv[] = {
+            {"ID", i},
+            {"infectionTime", e.infectedTimestamp},
+            {"infectedCount", e.infectedCount},
+            {"variant", e.variant},
+            {"InfectionLoc", e.infectedLocation},
+            {"diagnosisTime", e.diagnosedTimestamp},
+            {"quarantinedTime", e.quarantinedTimestamp},
+            {"quarantinedUntilTime", e.quarantinedUntilTimestamp},
+            {"daysInQuarantine", e.daysInQuarantine},
+            {"hospitalizedTime", e.hospitalizedTimestamp},
+            {"hospitalizedUntilTime", e.hospitalizedUntilTimestamp},
+            {"immunizationTimestamp", e.immunizationTimestamp},
+            {"immunizationCount", e.immunizationCount},
+            {"recoveredTimestamp", e.recoveredTimestamp},
+        };
+        for (const auto& p : kv) out << "            \"" << p.first << "\": " << p.second << ",\n";
+        out << "            \"worstState\": {\n";
+        out << "                \"name\": " << quoted(sc.st.names[(unsigned)(int)e.worstState]) << ",\n";
+        out << "                \"begin\": " << e.worstStateTimestamp << ",\n";
+        out << "                \"end\": ";
+        if (e.worstStateEndTimestamp == 0) out << "-1";
+        else out << e.worstStateEndTimestamp;
+        out << "\n            }\n        }";
+    }
+    out << (N ? "\n    ]\n}" : "]\n}");
+    f << out.str();
+}
+
+// The --diags block of printAgentStatJSON (agentsList.h:283-326).
+std::string Sim::agentStatDiags() const {
+    std::ostringstream os;
+    unsigned days = 0;
+    for (unsigned i = 0; i < N; ++i) days += stats[i].daysInQuarantine * (meta[i].age > 20 && meta[i].age <= 65);
+    os << "Total number of days in quarantine for agents 20<age<=65\n" << days << "\n";
+    const unsigned cemeteryLocation = L - 1;
+    std::vector<unsigned> types(N);
+    for (unsigned i = 0; i < N; ++i)
+        types[i] = locType[stats[i].infectedTimestamp == UMAX ? cemeteryLocation : stats[i].infectedLocation];
+    std::sort(types.begin(), types.end());
+    const size_t width = sc.lt.types.size() + 1 + 2;  // generalLocationTypes (with the cemetery) + 2
+    std::vector<unsigned> ids(width, 0u), counts(width, 0u);
+    size_t groups = 0;
+    for (unsigned i = 0; i < N; ++i) {
+        if (i == 0 || types[i] != types[i - 1]) {
+            if (groups < width) ids[groups] = types[i];
+            ++groups;
+        }
+        if (groups - 1 < width) counts[groups - 1]++;
+    }
+    for (unsigned x : ids) os << x << " ";
+    os << "\n";
+    for (unsigned x : counts) os << x << " ";
+    os << "\n";
+    return os.str();
+}
+
+// Simulation::finalize (simulation.h:958-961): printAgentStatJSON, then
+// BasicInfection::finalize (infectionPolicies.h:110-114).
+void Sim::finalize() {
+    writeAgentStatJSON(sc.opt.outAgentStat);
+    diagsOut = sc.opt.diags > 0 ? agentStatDiags() : std::string();
+    if (!sc.opt.dumpLocationInfectiousList.empty()) writeDump(sc.opt.dumpLocationInfectiousList);
+}
+
 }  // namespace oracle
 
 // ---------------------------------------------------------------------------
@@ -2044,6 +2197,25 @@ int oracle_run_for_day(void* h, int nflags, const char** flags, unsigned* out, i
         auto st = s->runForDay(f);
         for (int i = 0; i < cap && i < (int)st.size(); ++i) out[i] = st[i];
         return (int)st.size();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Writes the output files; copies what --diags printed into out (cap
+// bytes, NUL-terminated) and returns its length, or -1 on error.
+int oracle_finalize(void* h, char* out, int cap) {
+    auto* s = (oracle::Sim*)h;
+    try {
+        s->finalize();
+        const std::string& d = s->diagsOut;
+        if (out && cap > 0) {
+            const size_t n = std::min(d.size(), (size_t)cap - 1);
+            memcpy(out, d.data(), n);
+            out[n] = 0;
+        }
+        return (int)d.size();
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;
@@ -2141,6 +2313,10 @@ void oracle_location_types(void* h, unsigned* out) {
     for (unsigned l = 0; l < s->L; ++l) out[l] = s->locType[l];
 }
 unsigned oracle_num_states(void* h) { return ((oracle::Sim*)h)->nStates; }
+const char* oracle_state_name(void* h, unsigned k) {
+    auto* s = (oracle::Sim*)h;
+    return k < s->sc.st.names.size() ? s->sc.st.names[k].c_str() : "";
+}
 void oracle_state_table(void* h, unsigned char* wb, float* infectious) {
     auto* s = (oracle::Sim*)h;
     for (unsigned k = 0; k < s->nStates; ++k) {

@@ -32,6 +32,7 @@ def lib():
     L.oracle_destroy.argtypes = [ctypes.c_void_p]
     L.oracle_steps.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_run_for_day.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), u32p, ctypes.c_int]
+    L.oracle_finalize.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     for f in ("oracle_timestamp", "oracle_num_agents", "oracle_num_locations"):
         getattr(L, f).argtypes = [ctypes.c_void_p]
         getattr(L, f).restype = ctypes.c_uint32
@@ -47,6 +48,8 @@ def lib():
     L.oracle_location_types.argtypes = [ctypes.c_void_p, u32p]
     L.oracle_num_states.argtypes = [ctypes.c_void_p]
     L.oracle_num_states.restype = ctypes.c_uint32
+    L.oracle_state_name.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.oracle_state_name.restype = ctypes.c_char_p
     L.oracle_state_table.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)]
     L.oracle_agent_infectious.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
     dp = ctypes.POINTER(ctypes.c_double)
@@ -109,6 +112,15 @@ class OracleSim:
         if n < 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         return list(out[:n])
+
+    def finalize(self) -> str:
+        """Writes the output files (--outAgentStat, --dumpLocationInfectiousList)
+        and returns what --diags prints."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        n = lib().oracle_finalize(self._h, buf, len(buf))
+        if n < 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return buf.value.decode()
 
     @property
     def timestamp(self) -> int:
@@ -187,6 +199,13 @@ class OracleSim:
         lib().oracle_state_table(self._h, wb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                  inf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         return wb, inf
+
+    def state_names(self) -> List[str]:
+        return [lib().oracle_state_name(self._h, k).decode() for k in range(lib().oracle_num_states(self._h))]
+
+    def state_infectiousness_values(self) -> np.ndarray:
+        """The infectiousness of every state (variant multiplier 1)."""
+        return self.state_table()[1]
 
     def agent_infectious(self) -> np.ndarray:
         out = np.zeros(self.n_agents, dtype=np.float32)

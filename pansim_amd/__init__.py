@@ -60,6 +60,7 @@ def lib() -> ctypes.CDLL:
     L.pansim_sim_destroy.argtypes = [ctypes.c_void_p]
     L.pansim_sim_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     L.pansim_sim_run.argtypes = [ctypes.c_void_p]
+    L.pansim_sim_finalize.argtypes = [ctypes.c_void_p]
     L.pansim_sim_run_for_day.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), c_u32p,
                                          ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     for f in ("pansim_sim_timestamp", "pansim_sim_steps_per_day", "pansim_sim_total_steps"):
@@ -168,7 +169,13 @@ class Simulation:
         self._check(lib().pansim_sim_steps(self._h, n))
 
     def run(self):
+        """runSimulation: every step, then the output files."""
         self._check(lib().pansim_sim_run(self._h))
+
+    def finalize(self):
+        """The output files (--outAgentStat, --dumpLocationInfectiousList) and
+        the --diags summary of a simulation driven by steps or run_for_day."""
+        self._check(lib().pansim_sim_finalize(self._h))
 
     def run_for_day(self, flags: Sequence[str] = ()) -> List[int]:
         out = (ctypes.c_uint32 * 512)()
@@ -375,6 +382,21 @@ class SimulatorInterface:
         if self.sim is None:
             raise RuntimeError("initSimulation was not called")
         return self.sim.run_for_day(list(flags))
+
+    def close(self) -> None:
+        """The reference's destructor: finalize (output files), then release."""
+        if self.sim is not None:
+            try:
+                self.sim.finalize()
+            finally:
+                self.sim.close()
+                self.sim = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def config1_args(n_agents=100_000, n_locations=20_000, weeks=2, seed=0, extra: Sequence[str] = ()) -> List[str]:

@@ -131,6 +131,34 @@ __global__ void k_scatter_locations(Agents A, Locs L, Tables T) {
     }
 }
 
+// ------------------------------------------------------------ infectious-list dump
+// (BasicInfection::dumpLocationInfectiousList, infectionPolicies.h:262-441)
+// The locations of the agents infected since the last collection: flagged
+// in a bitmap, and recorded next to each agent.
+__global__ void k_dump_mark(Agents A, const uint32_t* agents, const uint32_t* count, uint32_t* locOut, uint32_t* flags) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *count) return;
+    const uint32_t l = A.location[agents[j]];
+    locOut[j] = l;
+    atomicOr(&flags[l >> 5], 1u << (l & 31u));
+}
+
+// Every agent with non-zero infectiousness at a flagged location.
+__global__ void k_dump_infectious(Agents A, Tables T, const uint32_t* flags, uint32_t* outAgent, uint32_t* outLoc,
+                                  float* outValue, uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const uint32_t l = A.location[i];
+    if (!((flags[l >> 5] >> (l & 31u)) & 1u)) return;
+    const AgentRec r = A.rec[i];
+    const float x = T.stInf[r.state] * T.varMul[r.variant];
+    if (x == 0.0f) return;
+    const uint32_t slot = atomicAdd(count, 1u);
+    outAgent[slot] = i;
+    outLoc[slot] = l;
+    outValue[slot] = x;
+}
+
 __global__ void k_count_variants(Agents A, Tables T, unsigned long long* out) {
     __shared__ unsigned int sh[MAXV];
     if (threadIdx.x < MAXV) sh[threadIdx.x] = 0;

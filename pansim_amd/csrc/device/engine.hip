@@ -50,6 +50,12 @@ struct pansim_ctx {
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
     bool hazardOn = false;
+    // infectious-list dump: the agents infected since the last collection,
+    // and the collection's buffers (allocated when the dump is enabled)
+    bool dumpOn = false;
+    uint32_t *dumpNew = nullptr, *dumpNewLoc = nullptr, *dumpCnt = nullptr, *dumpFlags = nullptr;
+    uint32_t *dumpInfAgent = nullptr, *dumpInfLoc = nullptr;
+    float* dumpInfVal = nullptr;
     uint32_t dt = 10;
     uint32_t wakeDay = 0;  // first step of the day the stored wake steps count from
 };
@@ -541,6 +547,8 @@ static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, 
     a.dt = (float)dt;
     a.typeMul = c->typeMul;
     a.hazard = c->hazardOn ? c->hazard : nullptr;
+    a.dumpList = c->dumpOn ? c->dumpNew : nullptr;
+    a.dumpCount = c->dumpCnt;
     a.resolve = 0;
     a.resolvePar = 0;
     a.nTiles = blocks_for(c->A.n, INFECT_TILE);
@@ -1100,6 +1108,61 @@ int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t
 }
 
 // ------------------------------------------------------------ hazard gradient
+int pansim_infection_dump_enable(pansim_ctx* c, uint32_t on) {
+    if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
+    if (on && !c->dumpNew) {
+        const size_t N = c->A.n, words = ((size_t)c->L.n + 31) / 32;
+        int r = 0;
+        if ((r = dalloc(c, &c->dumpNew, N)) || (r = dalloc(c, &c->dumpNewLoc, N)) || (r = dalloc(c, &c->dumpCnt, 4)) ||
+            (r = dalloc(c, &c->dumpFlags, words)) || (r = dalloc(c, &c->dumpInfAgent, N)) ||
+            (r = dalloc(c, &c->dumpInfLoc, N)) || (r = dalloc(c, &c->dumpInfVal, N))) {
+            c->dumpNew = nullptr;  // stays "never enabled" (the partial buffers are freed with the context)
+            return r;
+        }
+        HIP_OR_FAIL(c, hipMemsetAsync(c->dumpCnt, 0, 4 * sizeof(uint32_t), c->stream));
+        HIP_OR_FAIL(c, hipMemsetAsync(c->dumpFlags, 0, words * sizeof(uint32_t), c->stream));
+    }
+    c->dumpOn = on != 0;
+    return 0;
+}
+
+int pansim_infection_dump_collect(pansim_ctx* c, uint32_t* new_agent, uint32_t* new_loc, size_t cap_new, size_t* n_new,
+                                  uint32_t* inf_agent, uint32_t* inf_loc, float* inf_value, size_t cap_inf,
+                                  size_t* n_inf) {
+    if (!c || !n_new || !n_inf) return PANSIM_E_ARG;
+    if (!c->dumpNew) return fail(c, PANSIM_E_ARG, "infection_dump_collect: the dump was never enabled");
+    if (int fr = flush_resolve(c)) return fr;
+    const unsigned gN = blocks_for(c->A.n);
+    HIP_OR_FAIL(c, hipMemsetAsync(c->dumpCnt + 1, 0, sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_dump_mark, dim3(gN), dim3(256), 0, c->stream, c->A, c->dumpNew, c->dumpCnt, c->dumpNewLoc,
+                       c->dumpFlags);
+    hipLaunchKernelGGL(k_dump_infectious, dim3(gN), dim3(256), 0, c->stream, c->A, c->T, c->dumpFlags, c->dumpInfAgent,
+                       c->dumpInfLoc, c->dumpInfVal, c->dumpCnt + 1);
+    if (int r = check_launch(c, "infection_dump_collect")) return r;
+    uint32_t cnt[2];
+    HIP_OR_FAIL(c, hipMemcpyAsync(cnt, c->dumpCnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->dumpFlags, 0, ((size_t)c->L.n + 31) / 32 * sizeof(uint32_t), c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    *n_new = cnt[0];
+    *n_inf = cnt[1];
+    const bool query = !new_agent && !new_loc && !inf_agent && !inf_loc && !inf_value;
+    if (query) return 0;  // counts only; the record is kept
+    if (!new_agent || !new_loc || !inf_agent || !inf_loc || !inf_value || cap_new < cnt[0] || cap_inf < cnt[1])
+        return fail(c, PANSIM_E_ARG, "infection_dump_collect: output arrays missing or too small");
+    if (cnt[0]) {
+        HIP_OR_FAIL(c, hipMemcpy(new_agent, c->dumpNew, cnt[0] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_OR_FAIL(c, hipMemcpy(new_loc, c->dumpNewLoc, cnt[0] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    if (cnt[1]) {
+        HIP_OR_FAIL(c, hipMemcpy(inf_agent, c->dumpInfAgent, cnt[1] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_OR_FAIL(c, hipMemcpy(inf_loc, c->dumpInfLoc, cnt[1] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_OR_FAIL(c, hipMemcpy(inf_value, c->dumpInfVal, cnt[1] * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    HIP_OR_FAIL(c, hipMemsetAsync(c->dumpCnt, 0, sizeof(uint32_t), c->stream));
+    return 0;
+}
+
 int pansim_set_location_type_multipliers(pansim_ctx* c, const float* m, uint32_t n) {
     if (!c || (n && !m) || n > HAZARD_TYPES) return PANSIM_E_ARG;
     std::vector<float> v(HAZARD_TYPES, 1.0f);

@@ -725,6 +725,8 @@ struct InfectArgs {
     float dt;
     const float* typeMul;  // [HAZARD_TYPES] infectiousness multiplier of each location type (1 = reference)
     double* hazard;        // [HAZARD_SLOTS] accumulated expected infections and gradient, or null
+    uint32_t* dumpList;    // agents infected by this launch are appended here (infectious-list dump), or null
+    uint32_t* dumpCount;
     uint32_t resolve;      // also apply the preceding movement step's quarantine writes
     uint32_t resolvePar;   // their pending-counter parity
     uint32_t nTiles;       // infection blocks of this population (for the resolve stride)
@@ -781,6 +783,7 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
         a.A.worstState[i] = (int8_t)fi;
         a.A.worstTs[i] = a.timestamp;
         a.A.statVar[i] = (uint8_t)(a.A.statVar[i] | (1u << v));
+        if (a.dumpList) a.dumpList[atomicAdd(a.dumpCount, 1u)] = i;
     }
 }
 

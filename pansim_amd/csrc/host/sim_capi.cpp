@@ -76,6 +76,11 @@ int pansim_sim_run(pansim_sim* s) {
     return guarded(s, [&] { s->sim->runSimulation(); });
 }
 
+int pansim_sim_finalize(pansim_sim* s) {
+    if (!s) return PANSIM_E_ARG;
+    return guarded(s, [&] { s->sim->finalize(); });
+}
+
 int pansim_sim_run_for_day(pansim_sim* s, int nflags, const char* const* flags, uint32_t* out, size_t cap, size_t* len) {
     if (!s) return PANSIM_E_ARG;
     return guarded(s, [&] {
@@ -159,6 +164,7 @@ int pansim_ensemble_run(pansim_ensemble* e) {
     if (!e) return PANSIM_E_ARG;
     return guarded(e, [&] {
         while (e->ens->timestamp() < e->ens->totalSteps()) e->ens->step();
+        for (auto& s : e->ens->sims) s->printAgentStatJSON();  // each replica's runSimulation end
     });
 }
 

@@ -25,6 +25,7 @@
 
 #include "../../../include/pansim_hip.h"
 #include "model.h"
+#include "outputs.h"
 #include "scenario.h"
 
 namespace ps {
@@ -134,10 +135,43 @@ class BasicInfection {
     double c0_offset = 3.08, trunc = 0.5;
     std::vector<int> d_offset;
     int d_peak_offset = 0;
+    std::string dumpPath;  // --dumpLocationInfectiousList
+    bool dumpEnabled = false;
+    InfectiousListDump dump;
+
+    // One infection pass of one variant with the dump on: the launch, then
+    // the new infections and the infectious agents at their locations.
+    void infectAndDump(const Timehandler& simTime, unsigned timeStep, uint32_t variant, float tmpK, float season) {
+        auto* self = static_cast<SimulationType*>(this);
+        pansim_ctx* ctx = self->ctx;
+        if (!dumpEnabled) {
+            check(ctx, pansim_infection_dump_enable(ctx, 1), "dumpLocationInfectiousList");
+            dumpEnabled = true;
+        }
+        check(ctx, pansim_infection_step(ctx, simTime.getTimestamp(), 1u << variant, tmpK, season, timeStep),
+              "infectionsAtLocations");
+        size_t nNew = 0, nInf = 0;
+        check(ctx, pansim_infection_dump_collect(ctx, nullptr, nullptr, 0, &nNew, nullptr, nullptr, nullptr, 0, &nInf),
+              "dumpLocationInfectiousList");
+        std::vector<uint32_t> na(nNew + 1), nl(nNew + 1), ia(nInf + 1), il(nInf + 1);
+        std::vector<float> iv(nInf + 1);
+        check(ctx, pansim_infection_dump_collect(ctx, na.data(), nl.data(), na.size(), &nNew, ia.data(), il.data(), iv.data(),
+                                                 ia.size(), &nInf),
+              "dumpLocationInfectiousList");
+        na.resize(nNew);
+        nl.resize(nNew);
+        ia.resize(nInf);
+        il.resize(nInf);
+        iv.resize(nInf);
+        dump.insert(simTime.getTimestamp(), variant, na, nl, ia, il, iv);
+    }
 
 public:
     static void addProgramParameters(Options&) {}
-    void finalize() {}
+    // infectionPolicies.h:110-114
+    void finalize() {
+        if (!dumpPath.empty()) dump.write(dumpPath);
+    }
 
     void initializeArgs(const Options& o) {
         k = o.k;
@@ -145,7 +179,9 @@ public:
         d_peak_offset = o.d_peak_offset;
         c0_offset = o.c0;
         trunc = o.trunc;
+        dumpPath = o.dumpLocationInfectiousList;
     }
+    bool dumpsInfectiousList() const { return !dumpPath.empty(); }
 
     // infectionPolicies.h:443-478 (host, once per step)
     double seasonalityMultiplier(const Timehandler& simTime) const {
@@ -177,6 +213,11 @@ public:
         auto* self = static_cast<SimulationType*>(this);
         float tmpK = (float)k;
         float season = (float)seasonalityMultiplier(simTime);
+        if (!dumpPath.empty()) {  // one pass per variant, each dumped (infectionPolicies.h:548-557)
+            for (uint32_t v = 0; v < 32; ++v)
+                if ((mask >> v) & 1u) infectAndDump(simTime, timeStep, v, tmpK, season);
+            return;
+        }
         check(self->ctx, pansim_infection_step(self->ctx, simTime.getTimestamp(), mask, tmpK, season, timeStep),
               "infectionsAtLocations");
     }
@@ -724,7 +765,46 @@ public:
         if (!succesfullyInitialized) return;
         const unsigned total = lengthOfSimulationWeeks * 7u * simTime_.getStepsPerDay();
         while (simTime_.getTimestamp() < total) step();
+        printAgentStatJSON();
         Infection::finalize();
+    }
+
+    // simulation.h:958-961 (the day interface's end)
+    void finalize() {
+        if (!succesfullyInitialized) return;
+        printAgentStatJSON();
+        Infection::finalize();
+    }
+
+    // AgentList::printAgentStatJSON (agentsList.h:280-327): the --outAgentStat
+    // JSON, then with --diags > 0 the summary on stdout.
+    void printAgentStatJSON() {
+        if (sc.opt.outAgentStat.empty() && sc.opt.diags == 0) return;
+        AgentStatColumns c;
+        const uint32_t ts = simTime_.getTimestamp();
+        auto get = [&](const char* name, std::vector<uint32_t>& v) {
+            v.resize(sc.agents.n);
+            check(ctx, pansim_download_field(ctx, name, ts, v.data(), v.size()), "printAgentStatJSON");
+        };
+        get("infectedTimestamp", c.infectedTimestamp);
+        get("infectedCount", c.infectedCount);
+        get("statsVariant", c.variant);
+        get("infectedLocation", c.infectedLocation);
+        get("diagnosedTimestamp", c.diagnosedTimestamp);
+        get("quarantinedTimestamp", c.quarantinedTimestamp);
+        get("quarantinedUntilTimestamp", c.quarantinedUntilTimestamp);
+        get("daysInQuarantine", c.daysInQuarantine);
+        get("hospitalizedTimestamp", c.hospitalizedTimestamp);
+        get("hospitalizedUntilTimestamp", c.hospitalizedUntilTimestamp);
+        get("immunizationTimestamp", c.immunizationTimestamp);
+        get("immunizationCount", c.immunizationCount);
+        get("recoveredTimestamp", c.recoveredTimestamp);
+        get("worstState", c.worstState);
+        get("worstStateTimestamp", c.worstStateTimestamp);
+        get("worstStateEndTimestamp", c.worstStateEndTimestamp);
+        write_agent_stats_json(sc.opt.outAgentStat, c, sc.st.names);
+        if (sc.opt.diags > 0)
+            print_agent_stat_diags(std::cout, c, sc.agents.age, sc.locs.type, sc.locs.cemetery, sc.lt.types.size() + 1);
     }
 
     // simulation.h:928-956
