@@ -183,6 +183,16 @@ int pansim_infection_step(pansim_ctx* ctx, uint32_t timestamp, uint32_t variant_
  * agent states. */
 int pansim_plan_locations(pansim_ctx* ctx, const pansim_step_params* p);
 
+/* DummyMovement (movementPolicies.h:22-65): pansim_dummy_plan_locations is its
+ * planLocations (every countdown to 0; also rebuilds the per-location sums
+ * after the midnight passes); pansim_dummy_movement_step its movement (agents
+ * whose countdown is 0 move to a uniformly drawn location and draw a new
+ * countdown below 36; countdowns then decrement with the reference's unsigned
+ * wrap).  NoMovement needs no engine call beyond pansim_rebuild_locations at
+ * midnight. */
+int pansim_dummy_plan_locations(pansim_ctx* ctx);
+int pansim_dummy_movement_step(pansim_ctx* ctx, uint32_t timestamp);
+
 /* Recomputes the per-location occupancy and infectiousness sums from the
  * agents' current locations and states (after a midnight block that is not
  * followed by pansim_plan_locations, e.g. at the end of runForDay). */

@@ -305,6 +305,8 @@ public:
     double seasonality();
     void infectionsAtLocations(unsigned variant);
     void planLocations();
+    std::vector<unsigned> dummyStepsUntilMove;  // DummyMovement::stepsUntilMove
+    void dummyMovement();
     void performTests();
     void updateAgents();
     void immunizationUpdate();
@@ -888,7 +890,29 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
     sum--;
 }
 
+// DummyMovement::movement (movementPolicies.h:44-64): thrust::for_each over
+// the agents; randomUnsigned(numberOfLocations), then randomUnsigned(144/4),
+// then the unsigned decrement (an agent that draws 0 wraps and never moves
+// again: quirk Q6).
+void Sim::dummyMovement() {
+    const unsigned numberOfLocations = L;  // locationListOffsets.size() - 1
+    for (unsigned i = 0; i < N; ++i) {
+        unsigned& s = dummyStepsUntilMove[i];
+        if (s == 0) {
+            Rng rng(seed, i, timestamp, PHASE_DUMMY_MOVE);
+            location[i] = rng.below(numberOfLocations);
+            s = rng.below(144 / 4);
+        }
+        s--;
+    }
+}
+
 void Sim::movement() {
+    if (sc.opt.movementPolicy == "none") return;  // NoMovement::movement
+    if (sc.opt.movementPolicy == "dummy") {
+        dummyMovement();
+        return;
+    }
     TD simTime = TD::fromMinutes((timestamp % stepsPerDay) * dt);
     unsigned day = holidayModeActive ? 6u : dayOfWeek();
     std::vector<std::pair<unsigned, unsigned>> all, w;
@@ -1003,6 +1027,11 @@ void Sim::infectionsAtLocations(unsigned variant) {
 
 // movementPolicies.h:1428-1622
 void Sim::planLocations() {
+    if (sc.opt.movementPolicy == "none") return;  // NoMovement::planLocations
+    if (sc.opt.movementPolicy == "dummy") {       // DummyMovement::planLocations (:35-42)
+        dummyStepsUntilMove.assign(N, 0u);
+        return;
+    }
     if (!curfewTimeConverted) {
         curfewBegin = curfewBegin / dt;
         curfewEnd = curfewEnd / dt;

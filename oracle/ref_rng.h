@@ -83,7 +83,8 @@ enum : uint32_t {
     PHASE_MUTATION = 16,
     PHASE_VACCINATION = 17,
     PHASE_TEST_FLAG = 18,
-    PHASE_PLAN_UNDERAGE = 19
+    PHASE_PLAN_UNDERAGE = 19,
+    PHASE_DUMMY_MOVE = 20
 };
 
 inline float as_float(uint32_t u) {

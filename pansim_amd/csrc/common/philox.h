@@ -64,6 +64,7 @@ enum Phase : uint32_t {
     PH_VACCINATION = 17,   // vaccination / booster rollout
     PH_TEST_FLAG = 18,     // midnight testing: flagging locations
     PH_PLAN_UNDERAGE = 19, // midnight: children kept at home
+    PH_DUMMY_MOVE = 20,    // DummyMovement: random relocation
     PH_POP_LOCATION = 32,  // dummy population: locations
     PH_POP_AGENT = 33,     // dummy population: agents
     PH_SYNTH = 40          // synthetic household population builder

@@ -50,6 +50,7 @@ struct pansim_ctx {
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
     bool hazardOn = false;
+    uint32_t* dummyWait = nullptr;  // DummyMovement countdowns (allocated on first use)
     // infectious-list dump: the agents infected since the last collection,
     // and the collection's buffers (allocated when the dump is enabled)
     bool dumpOn = false;
@@ -607,6 +608,24 @@ int pansim_rebuild_locations(pansim_ctx* c) {
     if (!c) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
     return rebuild_locations(c);
+}
+
+int pansim_dummy_plan_locations(pansim_ctx* c) {
+    if (!c) return PANSIM_E_ARG;
+    if (int fr = flush_resolve(c)) return fr;
+    if (!c->dummyWait)
+        if (int r = dalloc(c, &c->dummyWait, c->A.n)) return r;
+    HIP_OR_FAIL(c, hipMemsetAsync(c->dummyWait, 0, (size_t)c->A.n * sizeof(uint32_t), c->stream));
+    return rebuild_locations(c);
+}
+
+int pansim_dummy_movement_step(pansim_ctx* c, uint32_t timestamp) {
+    if (!c) return PANSIM_E_ARG;
+    if (!c->dummyWait) return fail(c, PANSIM_E_ARG, "dummy_movement_step: pansim_dummy_plan_locations was never called");
+    if (int fr = flush_resolve(c)) return fr;
+    hipLaunchKernelGGL(k_dummy_move, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->L, c->T, c->dummyWait,
+                       c->seed, c->replica, timestamp);
+    return check_launch(c, "dummy_movement_step");
 }
 
 int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {

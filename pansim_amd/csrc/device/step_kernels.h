@@ -852,6 +852,39 @@ __global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
     infect_tile<HAZARD>(a, blockIdx.x);
 }
 
+// ------------------------------------------------------------ DummyMovement
+// DummyMovement::movement (movementPolicies.h:44-64): an agent whose
+// countdown is zero moves to a uniformly drawn location (any of the L,
+// cemetery included) and draws a new countdown below 36; every countdown
+// then decrements, wrapping from 0 to 2^32 - 1 (the reference's unsigned
+// underflow: an agent that draws 0 never moves again).  Occupancy and sums
+// are kept up to date as in the real movement.
+__global__ void k_dummy_move(Agents A, Locs L, Tables T, uint32_t* wait, uint64_t seed, uint32_t replica, uint32_t ts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    uint32_t w = wait[i];
+    if (w == 0u) {
+        Stream rs(seed, i, ts, PH_DUMMY_MOVE, replica);
+        const uint32_t loc = rs.uint_below(L.n);
+        w = rs.uint_below(144u / 4u);
+        const uint32_t old = A.location[i];
+        if (loc != old) {
+            A.location[i] = loc;
+            const AgentRec r = A.rec[i];
+            const float x = T.stInf[r.state] * T.varMul[r.variant];
+            if (x != 0.0f) {
+                const long long q = fixed32(x);
+                atomicAdd((unsigned long long*)loc_sum(L, r.variant, old), (unsigned long long)(-q));
+                atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)q);
+                mark_visited(L, loc);
+            }
+            atomicSub(&L.dyn[old].count, 1u);
+            atomicAdd(&L.dyn[loc].count, 1u);
+        }
+    }
+    wait[i] = w - 1u;
+}
+
 // ------------------------------------------------------------ replica batches
 // Many populations (replicas) stepped by one launch per kernel: every block
 // finds its replica in a per-block table and runs the same tile code on that

@@ -28,6 +28,8 @@ public:
                 sims.emplace_back(new ConfiguredSimulation(o, printOutput));
                 auto& s = *sims.back();
                 if (!s.succesfullyInitialized) throw InitError(s.initErrorCode ? s.initErrorCode : ERR_BAD_DATA, s.initError);
+                if (!s.realMovement())
+                    throw InitError(ERR_BAD_ARG, "ensemble replicas use the real movement policy");
                 if (s.dumpsInfectiousList())
                     throw InitError(ERR_BAD_ARG, "--dumpLocationInfectiousList is not available for ensemble replicas");
                 if (s.timeStep != sims.front()->timeStep)

@@ -91,6 +91,7 @@ std::vector<OptSpec> option_table(Options& o) {
         {"", "synthAgents", OptSpec::I, &o.synthAgents},
         {"", "synthSeed", OptSpec::U64, &o.synthSeed},
         {"", "synthInfected", OptSpec::D, &o.synthInfected},
+        {"", "movementPolicy", OptSpec::S, &o.movementPolicy},
         {"h", "help", OptSpec::B, &o.help},
         {"", "version", OptSpec::B, &o.version},
     };
@@ -175,6 +176,8 @@ Options parse_options(int argc, const char* const* argv, bool skip_program_name)
         }
         assign(*spec, spec->longName, value);
     }
+    if (o.movementPolicy != "real" && o.movementPolicy != "dummy" && o.movementPolicy != "none")
+        throw InitError(ERR_BAD_ARG, "bad value for --movementPolicy: " + o.movementPolicy + " (real, dummy or none)");
     return o;
 }
 

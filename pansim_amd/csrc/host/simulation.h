@@ -60,14 +60,37 @@ inline void check(pansim_ctx* ctx, int rc, const char* what) {
 }
 
 // ---------------------------------------------------------------- movement
+// NoMovement (include/policies/movementPolicies.h:9-20).  The midnight hook
+// rebuilds the engine's per-location sums, which the midnight passes change
+// (the reference recomputes them at every infection step).
 template <typename SimulationType>
 class NoMovement {
 public:
     static void addProgramParameters(Options&) {}
     void initializeArgs(const Options&) {}
     void init(const LocationTypeIds&, unsigned) {}
-    void planLocations(Timehandler, unsigned) {}
+    void planLocations(Timehandler, unsigned) {
+        auto* self = static_cast<SimulationType*>(this);
+        check(self->ctx, pansim_rebuild_locations(self->ctx), "planLocations");
+    }
     void movement(Timehandler, unsigned) {}
+};
+
+// DummyMovement (include/policies/movementPolicies.h:22-65)
+template <typename SimulationType>
+class DummyMovement {
+public:
+    static void addProgramParameters(Options&) {}
+    void initializeArgs(const Options&) {}
+    void init(const LocationTypeIds&, unsigned) {}
+    void planLocations(Timehandler, unsigned) {
+        auto* self = static_cast<SimulationType*>(this);
+        check(self->ctx, pansim_dummy_plan_locations(self->ctx), "planLocations");
+    }
+    void movement(Timehandler simTime, unsigned) {
+        auto* self = static_cast<SimulationType*>(this);
+        check(self->ctx, pansim_dummy_movement_step(self->ctx, simTime.getTimestamp()), "movement");
+    }
 };
 
 // RealMovement (include/policies/movementPolicies.h:1339-1752)
@@ -945,8 +968,36 @@ private:
     std::vector<uint32_t> offsetsHost_;
 };
 
-// The reference's concrete binding (cmake/config/in/configTypes.h.in:10-16).
-using ConfiguredSimulation = Simulation<std::array<double, 2>, unsigned, DynamicPPState, BasicAgentMeta, RealMovement,
-                                        BasicInfection, DetailedTesting, RuleClosure>;
+// RealMovement with the movement policy chosen at run time (--movementPolicy):
+// one library serves the three bindings the reference compiles separately.
+// The knobs the closure rules set stay those of RealMovement.
+template <typename SimulationType>
+class SelectableMovement : public RealMovement<SimulationType> {
+    using Real = RealMovement<SimulationType>;
+    enum Mode { REAL, DUMMY, NONE } mode_ = REAL;
+
+public:
+    void initializeArgs(const Options& o) {
+        Real::initializeArgs(o);
+        mode_ = o.movementPolicy == "dummy" ? DUMMY : o.movementPolicy == "none" ? NONE : REAL;
+    }
+    bool realMovement() const { return mode_ == REAL; }
+    void planLocations(Timehandler simTime, unsigned timeStep) {
+        auto* self = static_cast<SimulationType*>(this);
+        if (mode_ == REAL) Real::planLocations(simTime, timeStep);
+        else if (mode_ == DUMMY) check(self->ctx, pansim_dummy_plan_locations(self->ctx), "planLocations");
+        else check(self->ctx, pansim_rebuild_locations(self->ctx), "planLocations");
+    }
+    void movement(Timehandler simTime, unsigned timeStep) {
+        auto* self = static_cast<SimulationType*>(this);
+        if (mode_ == REAL) Real::movement(simTime, timeStep);
+        else if (mode_ == DUMMY) check(self->ctx, pansim_dummy_movement_step(self->ctx, simTime.getTimestamp()), "movement");
+    }
+};
+
+// The reference's concrete binding (cmake/config/in/configTypes.h.in:10-16),
+// with the movement policy selectable at run time.
+using ConfiguredSimulation = Simulation<std::array<double, 2>, unsigned, DynamicPPState, BasicAgentMeta,
+                                        SelectableMovement, BasicInfection, DetailedTesting, RuleClosure>;
 
 }  // namespace ps

@@ -19,7 +19,7 @@ def next_move_steps(sum_or_rel: np.ndarray, ts: int, steps_per_day: int) -> np.n
     return np.minimum(v, until_midnight)
 
 
-def compare_state(gpu: pansim_amd.Simulation, ora: OracleSim, where: str, fields=None):
+def compare_state(gpu: pansim_amd.Simulation, ora: OracleSim, where: str, fields=None, countdown=True):
     assert gpu.timestamp == ora.timestamp, where
     ts = gpu.timestamp
     mism = []
@@ -31,7 +31,7 @@ def compare_state(gpu: pansim_amd.Simulation, ora: OracleSim, where: str, fields
             mism.append(f"{f}: {len(idx)} differ, first {idx[:5].tolist()} gpu={a[idx[:5]].tolist()} oracle={b[idx[:5]].tolist()}")
     # countdown: the oracle stores stepsUntilMove, the GPU the absolute wake step
     spd = gpu.steps_per_day
-    if ts % spd != 0:
+    if countdown and ts % spd != 0:
         g = gpu.field("stepsUntilMove")
         o = ora.field("stepsUntilMove")
         g_next = next_move_steps(np.where(g == NEVER, np.int64(1) << 40, g.astype(np.int64)), ts, spd)
@@ -55,25 +55,27 @@ def compare_days(gpu, ora):
         assert a == b, f"day {d} statistics differ:\n gpu    {a}\n oracle {b}"
 
 
-def run_parity(args, checkpoints, flags_per_day=None):
+def run_parity(args, checkpoints, flags_per_day=None, countdown=True):
     """Steps both implementations through `checkpoints` (cumulative step
-    counts, or ("day", flags) for runForDay) comparing every field."""
+    counts, or ("day", flags) for runForDay) comparing every field
+    (countdown=False: not the RealMovement countdown, which the other
+    movement policies leave alone)."""
     gpu = pansim_amd.Simulation(args)
     ora = OracleSim(args)
     try:
-        compare_state(gpu, ora, "after init")
+        compare_state(gpu, ora, "after init", countdown=countdown)
         done = 0
         for cp in checkpoints:
             if isinstance(cp, tuple):
                 a = gpu.run_for_day(cp[1])
                 b = ora.run_for_day(cp[1])
                 assert a == b, f"runForDay {cp[1]} statistics differ:\n gpu    {a}\n oracle {b}"
-                compare_state(gpu, ora, f"after runForDay {cp[1]}")
+                compare_state(gpu, ora, f"after runForDay {cp[1]}", countdown=countdown)
                 continue
             gpu.steps(cp - done)
             ora.steps(cp - done)
             done = cp
-            compare_state(gpu, ora, f"after {cp} steps")
+            compare_state(gpu, ora, f"after {cp} steps", countdown=countdown)
         compare_days(gpu, ora)
         return gpu, ora
     except Exception:

@@ -76,3 +76,11 @@ def test_parity_vaccination_and_boosters():
     args = pansim_amd.config1_args(1500, 300, 19, 4, extra=[
         "-k", "0.005", "--immunizationStart", "2", "--immunizationsPerWeek", "700", "--boosterStart", "125"])
     run_parity(args, [289, 1008, 126 * 144 + 1, 19 * 1008])
+
+
+@pytest.mark.parametrize("policy", ["dummy", "none"])
+def test_parity_dummy_and_no_movement(policy):
+    # DummyMovement / NoMovement (movementPolicies.h:9-65) through
+    # --movementPolicy: locations, occupancy, sums and every statistic
+    args = _cfg(n=2500, nl=400, weeks=1, seed=13, k="0.05", extra=["--movementPolicy", policy])
+    run_parity(args, [1, 30, 144, 200, 1008], countdown=False)
