@@ -18,10 +18,11 @@ roofline: the per-step pipeline (movement kernel, then the infection kernel,
 which also applies the movement's deferred location quarantine writes) is
 the dominant work; its algorithmic bytes per step are
 22 B/agent + 24 B/location + 4 B/relocated agent, divided by its device time
-from one HIP event pair per step (before the movement launch, after the
-infection launch) on the context stream during the timed region.  The
-warmup runs with an event pair per kernel launch instead, for the
-movement / infection split.  traffic: HBM bytes per step of the
+from HIP events on the context stream during the timed region: one pair per
+day around the run of movement and infection launches between two midnight
+blocks (inter-kernel gaps included, midnight work excluded).  The warmup
+runs with an event pair per kernel launch instead, for the movement /
+infection split.  traffic: HBM bytes per step of the
 same kernels from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 (profiles/traffic_synth10M.json, 10M-agent workload only), else null.
 
@@ -230,7 +231,7 @@ def main():
     wm, wmove_ms, wi, winfect_ms = sim.timing_summary()
     log(f"warmup {a.warmup} steps done (t={sim.timestamp})")
     sim.movement_counters(reset=True)
-    # timed region: one event pair per step around movement + infection
+    # timed region: one event pair per day around its movement + infection launches
     sim.enable_timing(True, pipeline=True)
     sim.sync()
     t1 = time.perf_counter()

@@ -306,10 +306,13 @@ int pansim_infection_dump_collect(pansim_ctx* ctx, uint32_t* new_agent, uint32_t
  *   is bracketed by its own event pair; pansim_timing_summary returns the
  *   number of bracketed calls of each and their summed device time in
  *   milliseconds, pansim_last_kernel_ms the time of the last call of each.
- * PANSIM_TIMING_PIPELINE: one event pair per step, from the movement launch to
- *   the end of the infection launch that follows it (half the events of the
- *   kernel mode); pansim_timing_summary returns the pair count and summed time
- *   as n_move / move_ms and zero infection calls.
+ * PANSIM_TIMING_PIPELINE: one event pair around each uninterrupted run of
+ *   movement and infection launches (from the first movement launch after any
+ *   other entry point to the last launch before the next one, e.g. a day's
+ *   steps between two midnight blocks), so that the events add almost no
+ *   work; pansim_timing_summary returns the number of timed steps (movement
+ *   launches) and their summed device time as n_move / move_ms, and zero
+ *   infection calls.
  * pansim_enable_timing resets the record. */
 #define PANSIM_TIMING_OFF 0u
 #define PANSIM_TIMING_KERNELS 1u

@@ -222,13 +222,13 @@ class Simulation:
 
     def enable_timing(self, on: bool = True, pipeline: bool = False):
         """HIP-event timing of movement and infection: one event pair per call
-        (kernel mode) or, with pipeline=True, one pair per movement + infection
-        step."""
+        (kernel mode) or, with pipeline=True, one pair around each run of
+        movement + infection steps between other calls (a day's steps)."""
         self._check_ctx(lib().pansim_enable_timing(self.ctx, (2 if pipeline else 1) if on else 0))
 
     def timing_summary(self):
         """(n_movement_calls, movement_ms, n_infection_calls, infection_ms) since enable_timing;
-        in pipeline mode (n_steps, pipeline_ms, 0, 0)."""
+        in pipeline mode (n_steps, device ms of their movement + infection launches, 0, 0)."""
         nm, ni = ctypes.c_uint32(), ctypes.c_uint32()
         mm, mi = ctypes.c_double(), ctypes.c_double()
         self._check_ctx(lib().pansim_timing_summary(self.ctx, ctypes.byref(nm), ctypes.byref(mm), ctypes.byref(ni),

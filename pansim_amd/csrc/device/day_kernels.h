@@ -940,10 +940,23 @@ __global__ void k_set_open(Locs L, const uint32_t* types, uint32_t n, uint8_t op
     }
 }
 
-__global__ void k_scale_inf(Locs L, uint32_t homeType, double factor, uint32_t divide) {
+__global__ void k_scale_inf(Locs L, uint32_t homeType, double factor, uint32_t divide, const float* typeMul) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
+        const uint32_t t = L.rec[l].type;
+        if (t == homeType) continue;
+        const float v = divide ? (float)((double)L.infRaw[l] / factor) : (float)((double)L.infRaw[l] * factor);
+        L.infRaw[l] = v;
+        L.dyn[l].inf = v * typeMul[t];
+    }
+}
+
+// The infection factor of every location after a change of the type
+// multipliers.
+__global__ void k_loc_infm(Locs L, const float* typeMul) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride)
-        if (L.rec[l].type != homeType) L.dyn[l].inf = divide ? (float)((double)L.dyn[l].inf / factor) : (float)((double)L.dyn[l].inf * factor);
+        L.dyn[l].inf = L.infRaw[l] * typeMul[L.rec[l].type];
 }
 
 // ExposeToMutation (closurePolicies.h:356-390)

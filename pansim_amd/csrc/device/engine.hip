@@ -36,7 +36,8 @@ struct pansim_ctx {
     // pair per movement + infection step (pipeline mode, in moveEv)
     std::vector<hipEvent_t> moveEv, infectEv;
     size_t moveN = 0, infectN = 0;
-    bool pipeOpen = false;  // pipeline mode: a movement started a pair
+    bool pipeOpen = false;  // pipeline mode: a pair is open (a run of movement + infection launches)
+    size_t pipeSteps = 0;   // pipeline mode: movement launches inside the pairs
     unsigned long long* moveCounters = nullptr;  // MOVE_CTR_SLOTS pairs (evaluated movers, relocated agents)
     uint32_t pendPar = 0;                         // parity of the pending-write counter of the next step
     bool resolvePending = false;                  // a movement step's quarantine writes are not applied yet
@@ -50,6 +51,10 @@ struct pansim_ctx {
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
     bool hazardOn = false;
+    // per-variant infected counts of the last statistics pass, valid while no
+    // state-changing launch has run since (stateEpoch counts those)
+    uint64_t stateEpoch = 1, varEpoch = 0;
+    uint32_t varCache[MAXV] = {0};
     uint32_t* dummyWait = nullptr;  // DummyMovement countdowns (allocated on first use)
     // infectious-list dump: the agents infected since the last collection,
     // and the collection's buffers (allocated when the dump is enabled)
@@ -128,8 +133,33 @@ int check_launch(pansim_ctx* c, const char* what) {
 // launch has taken them (every entry point but pansim_infection_step calls
 // this first, so the location state it sees is complete), and ends an open
 // pipeline timing pair.
-static int flush_resolve(pansim_ctx* c) {
-    c->pipeOpen = false;  // other work between a movement and its infection: not timed as a pipeline step
+// Timing events: timestamps only, so no system-scope fence (cache
+// writeback) at each record.
+static int record_event(pansim_ctx* c, std::vector<hipEvent_t>& v, size_t idx) {
+    while (v.size() <= idx) {
+        hipEvent_t e;
+        HIP_OR_FAIL(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        v.push_back(e);
+    }
+    HIP_OR_FAIL(c, hipEventRecord(v[idx], c->stream));
+    return 0;
+}
+
+// Ends an open pipeline timing pair after the launches enqueued so far.
+static int pipe_close(pansim_ctx* c) {
+    if (!c->pipeOpen) return 0;
+    c->pipeOpen = false;
+    if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
+    c->moveN++;
+    return 0;
+}
+
+
+// keepPipe: called by pansim_movement_step, which continues an open
+// pipeline timing pair; every other entry point closes it.
+static int flush_resolve(pansim_ctx* c, bool keepPipe = false) {
+    if (!keepPipe)
+        if (int r = pipe_close(c)) return r;
     if (!c->resolvePending) return 0;
     c->resolvePending = false;
     hipLaunchKernelGGL(k_resolve_pending, dim3(64), dim3(256), 0, c->stream, c->L, c->S, c->resolvePar);
@@ -297,10 +327,11 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
             lr[k].flag = 0;
             ld[k].sum0 = 0;
             ld[k].count = 0;
-            ld[k].inf = l->infectiousness[k];
+            ld[k].inf = l->infectiousness[k];  // type multipliers start at 1
         }
         TRY(upload(c, &L.rec, lr.data(), NL));
         TRY(upload(c, &L.dyn, ld.data(), NL));
+        TRY(upload(c, &L.infRaw, l->infectiousness, NL));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
     TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));
@@ -447,23 +478,15 @@ int pansim_enable_timing(pansim_ctx* c, uint32_t on) {
     c->timing = on;
     c->moveN = c->infectN = 0;
     c->pipeOpen = false;
+    c->pipeSteps = 0;
     return 0;
 }
 
-// Timing events: timestamps only, so no system-scope fence (cache
-// writeback) at each record.
-static int record_event(pansim_ctx* c, std::vector<hipEvent_t>& v, size_t idx) {
-    while (v.size() <= idx) {
-        hipEvent_t e;
-        HIP_OR_FAIL(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-        v.push_back(e);
-    }
-    HIP_OR_FAIL(c, hipEventRecord(v[idx], c->stream));
-    return 0;
-}
+
 
 int pansim_timing_summary(pansim_ctx* c, uint32_t* n_move, double* move_ms, uint32_t* n_infect, double* infect_ms) {
     if (!c) return PANSIM_E_ARG;
+    if (int r = pipe_close(c)) return r;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     double m = 0.0, f = 0.0;
     for (size_t k = 0; k + 1 < 2 * c->moveN + 1 && 2 * k + 1 < c->moveEv.size() && k < c->moveN; ++k) {
@@ -476,7 +499,8 @@ int pansim_timing_summary(pansim_ctx* c, uint32_t* n_move, double* move_ms, uint
         HIP_OR_FAIL(c, hipEventElapsedTime(&x, c->infectEv[2 * k], c->infectEv[2 * k + 1]));
         f += x;
     }
-    if (n_move) *n_move = (uint32_t)c->moveN;
+    // pipeline mode: the steps (movement launches) the pairs cover
+    if (n_move) *n_move = (uint32_t)(c->timing == PANSIM_TIMING_PIPELINE ? c->pipeSteps : c->moveN);
     if (move_ms) *move_ms = m;
     if (n_infect) *n_infect = (uint32_t)c->infectN;
     if (infect_ms) *infect_ms = f;
@@ -559,13 +583,17 @@ static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, 
 
 int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     if (!c || !p) return PANSIM_E_ARG;
-    if (int fr = flush_resolve(c)) return fr;
+    if (int fr = flush_resolve(c, true)) return fr;
     if (p->time_step != c->dt) return fail(c, PANSIM_E_ARG, "movement_step: time step differs from the one given at creation");
     MoveArgs m = move_args(c, p);
     if (p->timestamp - m.stepOfDay != c->wakeDay)
         return fail(c, PANSIM_E_ARG, "movement_step: pansim_plan_locations must run at the start of every day");
     c->pendPar ^= 1u;
-    if (c->timing != PANSIM_TIMING_OFF && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+    if (c->timing == PANSIM_TIMING_KERNELS && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+    if (c->timing == PANSIM_TIMING_PIPELINE && !c->pipeOpen) {
+        if (record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+        c->pipeOpen = true;
+    }
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
     c->resolvePending = true;
     c->resolvePar = m.pendPar;
@@ -573,13 +601,14 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
         if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
         c->moveN++;
     }
-    c->pipeOpen = c->timing == PANSIM_TIMING_PIPELINE;
+    if (c->timing == PANSIM_TIMING_PIPELINE) c->pipeSteps++;
     return check_launch(c, "movement");
 }
 
 int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, float k, float season, uint32_t dt) {
     if (!c) return PANSIM_E_ARG;
     InfectArgs a = infect_args(c, timestamp, mask, k, season, dt);
+    if (a.mask) c->stateEpoch++;
     a.resolve = c->resolvePending ? 1u : 0u;
     a.resolvePar = c->resolvePar;
     c->resolvePending = false;
@@ -590,10 +619,6 @@ int pansim_infection_step(pansim_ctx* c, uint32_t timestamp, uint32_t mask, floa
     if (c->timing == PANSIM_TIMING_KERNELS) {
         if (record_event(c, c->infectEv, 2 * c->infectN + 1)) return PANSIM_E_DEVICE;
         c->infectN++;
-    } else if (c->pipeOpen) {  // closes the pair opened by the movement launch
-        if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
-        c->moveN++;
-        c->pipeOpen = false;
     }
     return check_launch(c, "infection");
 }
@@ -696,6 +721,7 @@ int pansim_testing_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const double pro
 int pansim_progression_day(pansim_ctx* c, uint32_t ts, uint32_t dt) {
     if (!c) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
+    c->stateEpoch++;
     hipLaunchKernelGGL(k_progression, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
                        ts, dt);
     return check_launch(c, "progression");
@@ -853,6 +879,7 @@ int pansim_vaccinate(pansim_ctx* c, uint32_t ts, uint32_t dt, uint32_t available
 int pansim_other_disease_day(pansim_ctx* c, uint32_t ts, uint32_t dt, float probabilityMul) {
     if (!c) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
+    c->stateEpoch++;
     hipLaunchKernelGGL(k_other_disease, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed,
                        c->replica, ts, dt, probabilityMul);
     return check_launch(c, "other_disease");
@@ -875,6 +902,10 @@ int pansim_count_hospitalized_covid(pansim_ctx* c, uint32_t ts, uint32_t dueWith
 int pansim_count_variants(pansim_ctx* c, uint32_t* out) {
     if (!c || !out) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
+    if (c->varEpoch == c->stateEpoch) {  // no state change since the statistics pass counted them
+        for (uint32_t v = 0; v < c->T.nVar; ++v) out[v] = c->varCache[v];
+        return 0;
+    }
     HIP_OR_FAIL(c, hipMemsetAsync(c->counters, 0, MAXV * sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_count_variants, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->counters);
     int r = check_launch(c, "count_variants");
@@ -882,7 +913,8 @@ int pansim_count_variants(pansim_ctx* c, uint32_t* out) {
     HIP_OR_FAIL(c, hipMemcpyAsync(c->hostCounters, c->counters, MAXV * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                   c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    for (uint32_t v = 0; v < c->T.nVar; ++v) out[v] = (uint32_t)c->hostCounters[v];
+    for (uint32_t v = 0; v < c->T.nVar; ++v) out[v] = c->varCache[v] = (uint32_t)c->hostCounters[v];
+    c->varEpoch = c->stateEpoch;
     return 0;
 }
 
@@ -899,6 +931,8 @@ int pansim_stats_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const uint32_t tes
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     const unsigned long long* h = c->hostCounters;
     const uint32_t NS = c->T.nStates, V = c->T.nVar;
+    for (uint32_t k = 0; k < V; ++k) c->varCache[k] = (uint32_t)h[S_VARINF + k];
+    c->varEpoch = c->stateEpoch;
     std::vector<uint32_t> v;
     for (uint32_t s = 0; s < NS; ++s) v.push_back((uint32_t)h[S_STATE + s]);
     v.push_back((uint32_t)h[S_HOSP]);
@@ -951,13 +985,15 @@ int pansim_set_open_by_type(pansim_ctx* c, const uint32_t* types, uint32_t n, ui
 int pansim_scale_infectiousness(pansim_ctx* c, double factor, uint32_t divide) {
     if (!c) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
-    hipLaunchKernelGGL(k_scale_inf, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.lt.home, factor, divide);
+    hipLaunchKernelGGL(k_scale_inf, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.lt.home, factor, divide,
+                       c->typeMul);
     return check_launch(c, "scale_infectiousness");
 }
 
 int pansim_expose_mutation(pansim_ctx* c, uint32_t ts, uint32_t v, double fraction) {
     if (!c || v >= c->T.nVar) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
+    c->stateEpoch++;
     hipLaunchKernelGGL(k_expose_mutation, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->T, c->seed, c->replica,
                        ts, v, fraction);
     return check_launch(c, "expose_mutation");
@@ -1068,7 +1104,7 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
     if (f == "locQuarantineUntil") return LOC_MEMBER(rec, LocRec, quarUntil);
     if (f == "locClosedUntil") return LOC_MEMBER(rec, LocRec, closedUntil);
     if (f == "locOpen") return LOC_MEMBER(rec, LocRec, open);
-    if (f == "locInfectiousness") return LOC_MEMBER(dyn, LocDyn, inf);
+    if (f == "locInfectiousness") return get32((const uint32_t*)c->L.infRaw, NL);
     if (f == "locCount") return LOC_MEMBER(dyn, LocDyn, count);
 #undef AGENT_MEMBER
 #undef LOC_MEMBER
@@ -1188,7 +1224,8 @@ int pansim_set_location_type_multipliers(pansim_ctx* c, const float* m, uint32_t
     for (uint32_t t = 0; t < n; ++t) v[t] = m[t];
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     HIP_OR_FAIL(c, hipMemcpy(c->typeMul, v.data(), HAZARD_TYPES * sizeof(float), hipMemcpyHostToDevice));
-    return 0;
+    hipLaunchKernelGGL(k_loc_infm, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->typeMul);
+    return check_launch(c, "set_location_type_multipliers");
 }
 
 int pansim_hazard_enable(pansim_ctx* c, uint32_t on) {
@@ -1318,6 +1355,7 @@ int pansim_batch_step(pansim_batch* b, uint32_t timestamp) {
         if (timestamp - stepOfDay != c->wakeDay)
             return batch_fail(b, PANSIM_E_ARG, "pansim_batch_step: pansim_plan_locations must run at the start of every day");
     const uint32_t n = (uint32_t)b->reps.size();
+    for (pansim_ctx* c : b->reps) c->stateEpoch++;
     BatchArgs a;
     a.move = b->dMove;
     a.infect = b->dInf;

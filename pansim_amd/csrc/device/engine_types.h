@@ -72,7 +72,8 @@ struct __align__(16) LocRec {
 static_assert(sizeof(LocRec) == 16, "LocRec is one 16-byte record");
 
 // Occupancy, exact variant-0 infectiousness sum (2^-32 fixed point) and
-// infectiousness of a location.
+// the location's infectiousness times its type's multiplier (the factor the
+// infection pass uses; Locs::infRaw holds the infectiousness itself).
 struct __align__(16) LocDyn {
     long long sum0;
     uint32_t count;
@@ -119,6 +120,7 @@ struct Locs {
     uint32_t cemetery;
     LocRec* rec;
     LocDyn* dyn;
+    float* infRaw;                  // [n] infectiousness (input, scaled by the closure rules)
     long long* sumX;                // [V-1][n]: sums of variants 1..V-1
     uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
     uint32_t* special;              // [(n + 31) / 32] bit l clear: l is open, not closed and not quarantined all day

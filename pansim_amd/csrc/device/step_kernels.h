@@ -755,17 +755,18 @@ __device__ __forceinline__ void hazard_terms(double* sh, float k, float density,
     atomicAdd(&sh[2 + type], w * ((double)k * (double)density * (double)inf * (double)season * (double)dt));
 }
 
+// infm: the location's infectiousness times its type's multiplier (LocDyn).
 template <bool HAZARD>
 __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
-                                             uint32_t count, float inf, float s, double* sh) {
+                                             uint32_t count, float infm, float s, double* sh) {
     const uint32_t n = a.A.n;
     float num = (float)((double)e * (1.0 / 4294967296.0));
     float density = num / (float)count;
-    const uint32_t type = a.L.rec[loc].type;
-    const float m = a.typeMul[type];
-    const float infm = inf * m;
     const float x = a.k * density * infm * a.season * a.dt;
-    if (HAZARD) hazard_terms(sh, a.k, density, inf, m, type, a.season, a.dt, x, s);
+    if (HAZARD) {
+        const uint32_t type = a.L.rec[loc].type;
+        hazard_terms(sh, a.k, density, a.L.infRaw[loc], a.typeMul[type], type, a.season, a.dt, x, s);
+    }
     float ratio = 1.0f - ps_expf(-x);
     if (ratio == 0.0f) return;
     Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
