@@ -276,10 +276,15 @@ __device__ __forceinline__ uint32_t div_dt(const MoveArgs& m, uint32_t x) {
 }
 
 constexpr uint32_t MOVE_BLOCK = 256;
-constexpr uint32_t MOVE_ITEMS = 16;
+// 8192 agents per block: 1221 blocks for 10M agents, one round of the
+// 256 CUs x 5 resident blocks, so that a step with few movers (latency
+// bound: one mover's dependent loads) costs one round, not two
+constexpr uint32_t MOVE_ITEMS = 32;
 constexpr uint32_t MOVE_TILE = MOVE_BLOCK * MOVE_ITEMS;  // agents per block
+constexpr uint32_t MOVE_CHUNKS = MOVE_ITEMS / 8;         // 16-byte wake vectors per thread
 constexpr uint32_t MOVE_CTR_SLOTS = 64;
-static_assert(MOVE_BLOCK == 256 && MOVE_ITEMS == 16, "the wake scan loads 2 x 8 wake steps per thread");
+static_assert(MOVE_ITEMS % 8 == 0, "the wake scan loads vectors of 8 wake steps");
+static_assert(MOVE_TILE <= 65536, "movers are listed as 16-bit offsets in the tile");
 
 __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, uint32_t loc, uint32_t until) {
     unsigned long long key = ((unsigned long long)agent << 32) | (unsigned long long)until;
@@ -635,7 +640,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
 // the location quarantine writes are deferred to k_resolve_pending.
 // Evaluated / relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
 __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t tile) {
-    __shared__ uint32_t list[MOVE_TILE];
+    __shared__ uint16_t list[MOVE_TILE];  // movers, as offsets in the tile
     __shared__ uint32_t nList, nMoved;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -645,20 +650,20 @@ __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t tile) {
         nList = 0;
         nMoved = 0;
     }
-    // two 16-byte loads per thread: agents base + k * (MOVE_TILE / 2) + tid * 8 + j
+    // MOVE_CHUNKS 16-byte loads per thread: agents base + k * (MOVE_TILE / MOVE_CHUNKS) + tid * 8 + j
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));
     const v4u* wv = reinterpret_cast<const v4u*>(m.A.wake + base);
-    v4u w[2];
+    v4u w[MOVE_CHUNKS];
 #pragma unroll
-    for (uint32_t k = 0; k < 2; ++k) w[k] = __builtin_nontemporal_load(&wv[k * (MOVE_TILE / 16) + tid]);
+    for (uint32_t k = 0; k < MOVE_CHUNKS; ++k) w[k] = __builtin_nontemporal_load(&wv[k * MOVE_BLOCK + tid]);
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (uint32_t k = 0; k < 2; ++k) {
+    for (uint32_t k = 0; k < MOVE_CHUNKS; ++k) {
         const uint32_t words[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
-            const uint32_t i = base + k * (MOVE_TILE / 2) + tid * 8u + j;
+            const uint32_t i = base + k * (MOVE_TILE / MOVE_CHUNKS) + tid * 8u + j;
             const uint32_t w16 = (words[j >> 1] >> (16u * (j & 1u))) & 0xFFFFu;
             bool mover = i < n && (m.curfewStep || w16 == m.stepOfDay);
             unsigned long long mask = __ballot(mover);
@@ -666,14 +671,14 @@ __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t tile) {
                 uint32_t pos = 0;
                 if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
                 pos = __shfl(pos, 0, 64);
-                if (mover) list[pos + (uint32_t)__popcll(mask & below)] = i;
+                if (mover) list[pos + (uint32_t)__popcll(mask & below)] = (uint16_t)(i - base);
             }
         }
     }
     __syncthreads();
     const uint32_t cnt = nList;
     uint32_t moved = 0;
-    for (uint32_t j = tid; j < cnt; j += MOVE_BLOCK) moved += do_movement(m, list[j]) ? 1u : 0u;
+    for (uint32_t j = tid; j < cnt; j += MOVE_BLOCK) moved += do_movement(m, base + list[j]) ? 1u : 0u;
     if (moved) atomicAdd(&nMoved, moved);
     __syncthreads();
     if (tid == 0 && cnt) {
