@@ -339,6 +339,7 @@ __global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replic
         int nd = calc_days(T, prog, next, v, rs);
         st = (uint8_t)next;
         A.rec[i].state = st;
+        A.rec[i].wb = T.stWB[st];
         update_meta(A, T, i, st);
         days = (int16_t)nd;
         if (v >= 3 && st < 8) {
@@ -660,6 +661,7 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
             // die(false) (dynamicPPState.cpp:313-325)
             uint8_t d = (uint8_t)T.nonCOVIDDead;
             A.rec[i].state = d;
+            A.rec[i].wb = T.stWB[d];
             A.days[i] = -1;
             update_meta(A, T, i, d);
             A.worstState[i] = (int8_t)d;
@@ -970,6 +972,7 @@ __global__ void k_expose_mutation(Agents A, Tables T, uint64_t seed, uint32_t re
     if (rs.unit() < fraction * (double)s) {
         uint8_t fi = (uint8_t)T.firstInfected;
         A.rec[i].state = fi;
+        A.rec[i].wb = T.stWB[fi];
         A.rec[i].variant = (uint8_t)v;
         A.days[i] = -2;
         update_meta(A, T, i, fi);

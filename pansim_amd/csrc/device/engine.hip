@@ -375,7 +375,7 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
             r.flags = (uint8_t)((a->diagnosed[k] ? F_DIAGNOSED : 0) | (a->healthcare_worker[k] ? F_HCW : 0));
             r.variant = 0;
             r.age = a->age[k];
-            r.pad = 0;
+            r.wb = t->state_wb[r.state];
             hcw += a->healthcare_worker[k] ? 1 : 0;
         }
         c->hcwCount = hcw;

@@ -54,7 +54,7 @@ struct __align__(32) AgentRec {
     uint8_t flags;       // AgentFlag bits
     uint8_t variant;
     uint8_t age;
-    uint8_t pad;
+    uint8_t wb;          // well-being class of the state (Tables::stWB[state]), written with every state
     uint16_t csrCount;   // length of the possible-location list
 };
 static_assert(sizeof(AgentRec) == 32, "AgentRec is one 32-byte record");

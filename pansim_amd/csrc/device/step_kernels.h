@@ -345,7 +345,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     uint32_t wake = wake0 == WAKE_NONE ? NEVER : dayStart + wake0;
     Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
     const uint8_t st = r.state;
-    const uint8_t w = T.stWB[st];
+    const uint8_t w = r.wb;  // T.stWB[st], kept in the record
     // state and variant are fixed during the routine
     const float xInf = T.stInf[st] * T.varMul[r.variant];
     const bool infectedState = T.stInfected[st] != 0;
@@ -779,6 +779,7 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
         // gotInfected (dynamicPPState.cpp:235-244) + stats (locationList.h:286-291)
         uint8_t fi = (uint8_t)a.T.firstInfected;
         a.A.rec[i].state = fi;
+        a.A.rec[i].wb = a.T.stWB[fi];
         a.A.rec[i].variant = (uint8_t)v;
         a.A.days[i] = -2;
         float sf = a.T.stSusc[fi] ? 1.0f : 0.0f;
