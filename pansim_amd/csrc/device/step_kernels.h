@@ -798,15 +798,9 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
 // (a separate instantiation, so that the plain pass carries no FP64 state).
 template <bool HAZARD>
 __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) {
-    // the movement step's deferred location writes: the infection pass never
-    // reads quarantine state, so they share its launch
-    if (a.resolve) resolve_pending(a.L, a.S, a.resolvePar, tile, a.nTiles);
-    if (a.mask == 0u) return;
-    __shared__ double shHazard[HAZARD ? HAZARD_SLOTS : 1];
-    double* sh = shHazard;
-    if (HAZARD) {
-        for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
-        __syncthreads();
+    if (a.mask == 0u) {
+        if (a.resolve) resolve_pending(a.L, a.S, a.resolvePar, tile, a.nTiles);
+        return;
     }
     // each wave takes 64 * INFECT_ITEMS consecutive agents, item k of lane
     // l being agent base + 64k: every load instruction covers 64 consecutive
@@ -825,6 +819,16 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
 #pragma unroll
     for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
         if ((a.L.visited[loc[k] >> 5] >> (loc[k] & 31u)) & 1u) cand |= 1u << k;
+    // the movement step's deferred location writes: the infection pass never
+    // reads quarantine state, so they share its launch, issued after the
+    // loads above so that the pending count's round trip overlaps them
+    if (a.resolve) resolve_pending(a.L, a.S, a.resolvePar, tile, a.nTiles);
+    __shared__ double shHazard[HAZARD ? HAZARD_SLOTS : 1];
+    double* sh = shHazard;
+    if (HAZARD) {
+        for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
+        __syncthreads();
+    }
     for (uint32_t v = 0; cand != 0u && v < a.T.nVar; ++v) {
         if (!((a.mask >> v) & 1u)) continue;
         // the location's sum of this variant, then the susceptibility, which
@@ -854,8 +858,13 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
     }
 }
 
+// amdgpu_waves_per_eu(8): the plain pass is bound by the latency of its
+// dependent loads, so it asks for the most resident waves (the compiler then
+// keeps it within 96 SGPRs, moving rarely used pointers into VGPR lanes);
+// the hazard pass, with its FP64 terms, is left to the compiler's choice
 template <bool HAZARD>
-__global__ void __launch_bounds__(INFECT_BLOCK) k_infect(InfectArgs a) {
+__global__ void __launch_bounds__(INFECT_BLOCK) __attribute__((amdgpu_waves_per_eu(HAZARD ? 1 : 8)))
+k_infect(InfectArgs a) {
     infect_tile<HAZARD>(a, blockIdx.x);
 }
 
@@ -918,7 +927,8 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_batch(BatchArgs b) {
 }
 
 template <bool HAZARD>
-__global__ void __launch_bounds__(INFECT_BLOCK) k_infect_batch(BatchArgs b) {
+__global__ void __launch_bounds__(INFECT_BLOCK) __attribute__((amdgpu_waves_per_eu(HAZARD ? 1 : 8)))
+k_infect_batch(BatchArgs b) {
     const uint32_t r = b.infRep[blockIdx.x];
     InfectArgs a = b.infect[r];
     a.timestamp = b.ts;
