@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from typing import List, Sequence
 
 import numpy as np
@@ -367,21 +368,32 @@ class EnsembleReplica:
 
 class SimulatorInterface:
     """Mirror of the reference's pybind11 pyPanSim.SimulatorInterface
-    (matlab/mexPanSim.cpp:57-127, 207-219)."""
+    (matlab/mexPanSim.cpp:57-127, 207-219); the compiled module of that name
+    is pyPanSim().  As there, runForDay with an empty flag list runs nothing
+    and returns the previous day's statistics (mexPanSim.cpp:113-114), and
+    runForDay before initSimulation prints a message and returns []."""
 
     def __init__(self):
         self.sim = None
+        self._stats: List[int] = []
 
     def initSimulation(self, args: Sequence[str]) -> None:
         args = list(args)
         if args and not str(args[0]).startswith("-"):
             args = args[1:]  # the reference receives argv including the program name
+        self.close()
         self.sim = Simulation(args)
+        self._stats = []
 
-    def runForDay(self, flags: Sequence[str] = ()) -> List[int]:
+    def runForDay(self, flags: Sequence[str]) -> List[int]:
         if self.sim is None:
-            raise RuntimeError("initSimulation was not called")
-        return self.sim.run_for_day(list(flags))
+            print("Cannot run uninitialized simulation... Initialize it first.", end="", flush=True)
+            return []
+        flags = list(flags)
+        if not flags:
+            return self._stats
+        self._stats = self.sim.run_for_day(flags)
+        return self._stats
 
     def close(self) -> None:
         """The reference's destructor: finalize (output files), then release."""
@@ -397,6 +409,17 @@ class SimulatorInterface:
             self.close()
         except Exception:
             pass
+
+
+def pyPanSim():
+    """The pybind11 module pyPanSim (the reference's binding name, built
+    next to libpansim.so): ``import pyPanSim`` works once
+    ``pansim_amd/_lib`` is on sys.path; this returns the module either way."""
+    lib = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
+    if lib not in sys.path:
+        sys.path.insert(0, lib)
+    import importlib
+    return importlib.import_module("pyPanSim")
 
 
 def config1_args(n_agents=100_000, n_locations=20_000, weeks=2, seed=0, extra: Sequence[str] = ()) -> List[str]:

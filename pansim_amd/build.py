@@ -12,6 +12,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+import sysconfig
 import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -67,8 +68,22 @@ def build_engine(verbose=False):
     _run([CXX, *CXXFLAGS, "-c", os.path.join(CSRC, "host", "main.cpp"), "-o", main_obj], verbose)
     exe = os.path.join(LIBDIR, "panSim")
     _run([CXX, "-o", exe, main_obj, f"-L{LIBDIR}", "-lpansim", "-Wl,-rpath,$ORIGIN"], verbose)
+    build_pypansim(verbose)
     shutil.rmtree(objdir, ignore_errors=True)
     return lib
+
+
+def build_pypansim(verbose=False):
+    """The pybind11 module pyPanSim (the reference's Python binding) over
+    libpansim.so, next to it in _lib/."""
+    import pybind11
+    out = os.path.join(LIBDIR, "pyPanSim" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    _run([CXX, *CXXFLAGS, "-fvisibility=hidden", "-shared", f"-I{pybind11.get_include()}",
+          f"-I{sysconfig.get_paths()['include']}", os.path.join(CSRC, "host", "pypansim.cpp"), "-o", tmp,
+          f"-L{LIBDIR}", "-lpansim", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle(verbose=False):

@@ -40,13 +40,13 @@ def test_output_files_through_run_for_day(tmp_path, capfd):
     si = pansim_amd.SimulatorInterface()
     si.initSimulation(pansim_amd.with_data_dir(_args(tmp_path, "gpu")))
     for _ in range(3):
-        si.runForDay([])
+        si.runForDay(["TPdef"])
     capfd.readouterr()
     si.close()
     gpu_diags = capfd.readouterr().out
     o = OracleSim(pansim_amd.with_data_dir(_args(tmp_path, "cpu")))
     for _ in range(3):
-        o.run_for_day([])
+        o.run_for_day(["TPdef"])
     cpu_diags = o.finalize()
     assert (tmp_path / "agents_gpu.json").read_text() == (tmp_path / "agents_cpu.json").read_text()
     assert (tmp_path / "infectious_gpu.txt").read_text() == (tmp_path / "infectious_cpu.txt").read_text()
