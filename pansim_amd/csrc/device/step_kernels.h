@@ -837,10 +837,26 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
         const long long* sv = v == 0 ? &a.L.dyn[0].sum0 : a.L.sumX + (size_t)(v - 1) * a.L.n;
         const uint32_t pitch = v == 0 ? (uint32_t)(sizeof(LocDyn) / sizeof(long long)) : 1u;
         const float* su = a.A.susc + (size_t)v * a.A.suscStride + base;
+        // two rounds of loads, each issued for all items before any result
+        // is used, so that a wave waits for two memory round trips rather
+        // than two per item: the sums at the candidates' locations, then the
+        // susceptibility of the agents where the sum is not zero
+        long long e[INFECT_ITEMS];
+        float s[INFECT_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
+            e[k] = 0;
+            if ((cand >> k) & 1u) e[k] = sv[(size_t)loc[k] * pitch];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
+            s[k] = 0.0f;
+            if (e[k] != 0) s[k] = su[64u * k];
+        }
         uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
-            if (((cand >> k) & 1u) && sv[(size_t)loc[k] * pitch] != 0 && su[64u * k] > 0.0f) hits |= 1u << k;
+            if (s[k] > 0.0f) hits |= 1u << k;
         // the rare agents at a location with infectious visitors: one at a time
         while (hits) {
             const uint32_t k = (uint32_t)__builtin_ctz(hits);
@@ -848,7 +864,7 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
             const uint32_t i = base + 64u * k;
             const uint32_t l = loc[k];
             const LocDyn d = a.L.dyn[l];
-            infect_agent<HAZARD>(a, i, v, l, sv[(size_t)l * pitch], d.count, d.inf, su[64u * k], sh);
+            infect_agent<HAZARD>(a, i, v, l, e[k], d.count, d.inf, s[k], sh);
         }
     }
     if (HAZARD) {
