@@ -846,13 +846,15 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
             e[k] = 0;
-            if ((cand >> k) & 1u) e[k] = sv[(size_t)loc[k] * pitch];
+            s[k] = 0.0f;
+            if ((cand >> k) & 1u) {
+                e[k] = sv[(size_t)loc[k] * pitch];
+                s[k] = su[64u * k];
+            }
         }
 #pragma unroll
-        for (uint32_t k = 0; k < INFECT_ITEMS; ++k) {
-            s[k] = 0.0f;
-            if (e[k] != 0) s[k] = su[64u * k];
-        }
+        for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
+            if (e[k] == 0) s[k] = 0.0f;
         uint32_t hits = 0;
 #pragma unroll
         for (uint32_t k = 0; k < INFECT_ITEMS; ++k)
