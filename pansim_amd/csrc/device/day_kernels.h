@@ -211,7 +211,10 @@ __global__ void k_flag_locations(TestArgs a) {
     }
 }
 
-// DetailedTestingOps::doTesting (testingPolicies.h:157-281) + the counts of performTests
+// DetailedTestingOps::doTesting (testingPolicies.h:157-281) + the counts of performTests.
+// The agent's record is read once, and the location records of its home,
+// workplace and school are loaded together, so that a thread waits for a
+// few memory round trips instead of one per field.
 __global__ void k_do_testing(TestArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ unsigned int sh[3];
@@ -222,20 +225,32 @@ __global__ void k_do_testing(TestArgs a) {
         const uint32_t day = 24u * 60u / a.dt;
         const uint32_t ts = a.ts;
         const pansim_location_types& lt = a.T.lt;
-        uint8_t fl = A.rec[i].flags;
-        bool skip = ts > a.delay * day && A.lastTest[i] != NEVER && A.lastTest[i] > ts - a.delay * day;
-        uint8_t st = A.rec[i].state;
-        if (!skip && !(a.T.stWB[st] == 4 || (fl & F_DIAGNOSED))) {
+        const AgentRec r = A.rec[i];
+        uint32_t lastTest = A.lastTest[i];
+        uint32_t diagTs = r.diagTs;
+        const uint8_t fl = r.flags;
+        const bool skip = ts > a.delay * day && lastTest != NEVER && lastTest > ts - a.delay * day;
+        const uint8_t st = r.state;
+        if (!skip && !(r.wb == 4 || (fl & F_DIAGNOSED))) {  // wb: T.stWB[st], kept in the record
             Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
-            const CsrReg csr = load_csr(A, i, A.rec[i]);
-            uint32_t home = csr.home;
-            bool homeFlag = home != NOLOC ? a.L.rec[home].flag != 0 : false;
-            uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
-            bool workFlag = work != NOLOC ? a.L.rec[work].flag != 0 : false;
-            uint32_t school = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
+            const CsrReg csr = load_csr(A, i, r);
+            const uint32_t home = csr.home;
+            const uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
+            const uint32_t school = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
+            LocRec hr, wr;
+            hr.flag = 0;
+            hr.quarUntil = 0;
+            wr.flag = 0;
+            wr.type = LOC_TYPE_NONE;
+            uint8_t schoolF = 0;
+            if (home != NOLOC) hr = a.L.rec[home];
+            if (work != NOLOC) wr = a.L.rec[work];
+            if (school != NOLOC) schoolF = a.L.rec[school].flag;
+            const bool homeFlag = hr.flag != 0;
+            const bool workFlag = wr.flag != 0;
             bool schoolFlag = false, classFlag = false;
             if (school != NOLOC) {
-                schoolFlag = a.L.rec[school].flag != 0;
+                schoolFlag = schoolF != 0;
                 uint32_t cls = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
                 if (cls != NOLOC) classFlag = true;
             }
@@ -244,29 +259,30 @@ __global__ void k_do_testing(TestArgs a) {
             prob += (double)workFlag * a.pWork;
             prob += (double)schoolFlag * a.pSchool;
             prob += (double)classFlag * 3.0 * a.pSchool;
-            if (work != NOLOC && (a.L.rec[work].type == lt.doctor || a.L.rec[work].type == lt.hospital)) prob += a.pHospital;
-            if (work != NOLOC && a.L.rec[work].type == lt.nursery_home) prob += a.pNursery;
-            if (A.rec[i].hospTs <= ts && A.rec[i].hospUntil > ts) prob += a.pHospital;
+            if (work != NOLOC && (wr.type == lt.doctor || wr.type == lt.hospital)) prob += a.pHospital;
+            if (work != NOLOC && wr.type == lt.nursery_home) prob += a.pNursery;
+            if (r.hospTs <= ts && r.hospUntil > ts) prob += a.pHospital;
             if (prob > 1.0 || rs.real(1.0) < prob) {
                 A.lastTest[i] = ts;
+                lastTest = ts;
                 if (a.T.stInfected[st]) {
                     float pr = a.usePCR ? a.T.accPCR[st] : a.T.accAnt[st];
                     if ((double)pr > rs.real(1.0)) {
                         A.rec[i].flags = (uint8_t)(fl | F_DIAGNOSED);
                         A.rec[i].diagTs = ts;
+                        diagTs = ts;
                     }
                 } else {
-                    if (A.rec[i].quarUntil > ts && (home != NOLOC && a.L.rec[home].quarUntil < ts)) {
-                        if (A.daysInQ[i] > 0) A.daysInQ[i] -= (A.rec[i].quarUntil - ts) / day;
+                    if (r.quarUntil > ts && (home != NOLOC && hr.quarUntil < ts)) {
+                        if (A.daysInQ[i] > 0) A.daysInQ[i] -= (r.quarUntil - ts) / day;
                         A.rec[i].quarUntil = ts;
                     }
                 }
             }
         }
-        if (A.lastTest[i] == ts) atomicAdd(&sh[0], 1u);
-        uint32_t d = A.rec[i].diagTs;
-        if (d == ts) atomicAdd(&sh[1], 1u);
-        if (d < ts && d > ts - day) atomicAdd(&sh[2], 1u);
+        if (lastTest == ts) atomicAdd(&sh[0], 1u);
+        if (diagTs == ts) atomicAdd(&sh[1], 1u);
+        if (diagTs < ts && diagTs > ts - day) atomicAdd(&sh[2], 1u);
     }
     __syncthreads();
     if (threadIdx.x < 3 && sh[threadIdx.x]) atomicAdd(&a.counters[threadIdx.x], (unsigned long long)sh[threadIdx.x]);
@@ -836,11 +852,15 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
         uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
         uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
         uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
+        // the three quarantine times loaded together, then the first that applies
         const LocRec* q = a.L.rec;
+        const uint32_t qs = sl != NOLOC ? q[sl].quarUntil : 0u;
+        const uint32_t qc = cl != NOLOC ? q[cl].quarUntil : 0u;
+        const uint32_t qw = wl != NOLOC ? q[wl].quarUntil : 0u;
         uint32_t until = 0;
-        if (sl != NOLOC && q[sl].quarUntil > a.ts) until = q[sl].quarUntil;
-        else if (cl != NOLOC && q[cl].quarUntil > a.ts) until = q[cl].quarUntil;
-        else if (wl != NOLOC && q[wl].quarUntil > a.ts) until = q[wl].quarUntil;
+        if (qs > a.ts) until = qs;
+        else if (qc > a.ts) until = qc;
+        else if (qw > a.ts) until = qw;
         if (until > 0) {
             fl |= F_QUARANTINED;
             A.quarTs[i] = a.ts;
