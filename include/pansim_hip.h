@@ -280,7 +280,9 @@ int pansim_upload_schedules(pansim_ctx* ctx, const uint32_t* event_offset, uint3
  * raw bits): "state", "variant", "days", "location", "stepsUntilMove",
  * "diagnosed", "quarantined", "stayedHome", "noWorkAgent", "lastTest",
  * "susceptible0", AgentStats members, "vaccinationRound", "locQuarantineUntil",
- * "locClosedUntil", "locOpen", "locInfectiousness", "locCount".
+ * "locClosedUntil", "locOpen", "locInfectiousness", "locCount", and the
+ * possible-location lists: "csrBegin", "csrCount" (per agent) and
+ * "possibleLocations" (all entries, n >= their number).
  * "stepsUntilMove" is reported relative to `timestamp`. */
 int pansim_download_field(pansim_ctx* ctx, const char* name, uint32_t timestamp, uint32_t* out, size_t n);
 

@@ -2353,6 +2353,11 @@ void oracle_state_table(void* h, unsigned char* wb, float* infectious) {
         infectious[k] = s->infTable[k];
     }
 }
+unsigned oracle_variant_multipliers(void* h, float* out, unsigned cap) {
+    auto* s = (oracle::Sim*)h;
+    for (unsigned v = 0; v < s->nVar && v < cap; ++v) out[v] = s->variantMul[v];
+    return s->nVar;
+}
 void oracle_agent_infectious(void* h, float* out) {
     auto* s = (oracle::Sim*)h;
     for (unsigned i = 0; i < s->N; ++i) out[i] = s->pp[i].infectious;

@@ -52,6 +52,8 @@ def lib():
     L.oracle_state_name.restype = ctypes.c_char_p
     L.oracle_state_table.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_float)]
     L.oracle_agent_infectious.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+    L.oracle_variant_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
+    L.oracle_variant_multipliers.restype = ctypes.c_uint32
     dp = ctypes.POINTER(ctypes.c_double)
     L.oracle_set_type_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
     L.oracle_hazard_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -206,6 +208,12 @@ class OracleSim:
     def state_infectiousness_values(self) -> np.ndarray:
         """The infectiousness of every state (variant multiplier 1)."""
         return self.state_table()[1]
+
+    def variant_multipliers(self) -> np.ndarray:
+        """The infectiousness multiplier of every variant."""
+        out = np.zeros(16, dtype=np.float32)
+        n = lib().oracle_variant_multipliers(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 16)
+        return out[:n]
 
     def agent_infectious(self) -> np.ndarray:
         out = np.zeros(self.n_agents, dtype=np.float32)

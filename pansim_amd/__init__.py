@@ -221,6 +221,18 @@ class Simulation:
             raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
         return out
 
+    def possible_locations(self):
+        """(first entry, number of entries) of every agent's possible-location
+        list, and all list entries, as held on the device."""
+        begin, count = self.field("csrBegin"), self.field("csrCount")
+        total = int(begin[-1]) + int(count[-1]) if self.n_agents else 0
+        out = np.zeros(max(1, total), dtype=np.uint32)
+        rc = lib().pansim_download_field(self.ctx, b"possibleLocations", self.timestamp,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(out))
+        if rc != 0:
+            raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
+        return begin, count, out[:total]
+
     def enable_timing(self, on: bool = True, pipeline: bool = False):
         """HIP-event timing of movement and infection: one event pair per call
         (kernel mode) or, with pipeline=True, one pair around each run of

@@ -47,6 +47,7 @@ struct pansim_ctx {
     pansim_event* evDev = nullptr;
     uint32_t evOffCap = 0, evCap = 0;
     uint32_t nEvents = 0;
+    uint32_t nCsr = 0;  // entries of the possible-location lists
     int2* evWinDev = nullptr;
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
@@ -413,6 +414,7 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(dalloc(c, &A.infCount, N));
     TRY(dalloc(c, &A.worstState, N));
     TRY(upload(c, (uint32_t**)&A.possLoc, a->possible_locations, d->n_csr));
+    c->nCsr = d->n_csr;
     {
         TRY(upload(c, (uint8_t**)&A.possType, a->possible_types, d->n_csr));
         // the types of every list of at most 16 entries as one 16-byte record per agent
@@ -1015,10 +1017,12 @@ int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_
     if (!c || !out || v >= c->T.nVar || n < c->L.n) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    if (v == 0)
-        HIP_OR_FAIL(c, hipMemcpy2D(out, sizeof(int64_t), &c->L.dyn[0].sum0, sizeof(LocDyn), sizeof(int64_t), c->L.n,
-                                   hipMemcpyDeviceToHost));
-    else
+    if (v == 0) {
+        // the LocDyn records copied whole, the sums picked out on the host
+        std::vector<LocDyn> tmp(c->L.n);
+        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), c->L.dyn, tmp.size() * sizeof(LocDyn), hipMemcpyDeviceToHost));
+        for (uint32_t l = 0; l < c->L.n; ++l) out[l] = tmp[l].sum0;
+    } else
         HIP_OR_FAIL(c, hipMemcpy(out, c->L.sumX + (size_t)(v - 1) * c->L.n, c->L.n * sizeof(int64_t), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -1041,14 +1045,21 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
         for (uint32_t i = 0; i < cnt; ++i) out[i] = isSigned ? (uint32_t)(int32_t)(int8_t)tmp[i] : tmp[i];
         return 0;
     };
-    // one member of an array of records: a strided copy
+    // one member of an array of records: the records are copied whole (one
+    // contiguous transfer; a strided 2D copy of millions of rows is slow)
+    // and the member is picked out on the host
     auto getMember = [&](const void* base, size_t pitch, size_t offset, size_t width, uint32_t cnt) -> int {
         if (n < cnt) return fail(c, PANSIM_E_ARG, "output buffer too small");
-        std::vector<uint8_t> tmp((size_t)cnt * width);
-        HIP_OR_FAIL(c, hipMemcpy2D(tmp.data(), width, (const uint8_t*)base + offset, pitch, width, cnt, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> tmp((size_t)cnt * pitch);
+        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), base, tmp.size(), hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < cnt; ++i) {
-            if (width == 4) memcpy(&out[i], &tmp[(size_t)i * 4], 4);
-            else out[i] = tmp[i];
+            const uint8_t* p = &tmp[(size_t)i * pitch + offset];
+            if (width == 4) memcpy(&out[i], p, 4);
+            else if (width == 2) {
+                uint16_t v;
+                memcpy(&v, p, 2);
+                out[i] = v;
+            } else out[i] = *p;
         }
         return 0;
     };
@@ -1071,6 +1082,9 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
         return 0;
     }
     if (f == "location") return get32(A.location, N);
+    if (f == "csrBegin") return AGENT_MEMBER(csrBegin);
+    if (f == "csrCount") return AGENT_MEMBER(csrCount);
+    if (f == "possibleLocations") return get32(A.possLoc, c->nCsr);
     if (f == "stepsUntilMove") {
         if (n < N) return fail(c, PANSIM_E_ARG, "output buffer too small");
         std::vector<uint16_t> tmp(N);
