@@ -837,10 +837,10 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
         const long long* sv = v == 0 ? &a.L.dyn[0].sum0 : a.L.sumX + (size_t)(v - 1) * a.L.n;
         const uint32_t pitch = v == 0 ? (uint32_t)(sizeof(LocDyn) / sizeof(long long)) : 1u;
         const float* su = a.A.susc + (size_t)v * a.A.suscStride + base;
-        // two rounds of loads, each issued for all items before any result
-        // is used, so that a wave waits for two memory round trips rather
-        // than two per item: the sums at the candidates' locations, then the
-        // susceptibility of the agents where the sum is not zero
+        // one round of loads for all items, issued before any result is used
+        // (a wave waits for one memory round trip, not two per item): the
+        // sums at the candidates' locations and the candidates'
+        // susceptibilities
         long long e[INFECT_ITEMS];
         float s[INFECT_ITEMS];
 #pragma unroll
