@@ -84,3 +84,14 @@ def test_parity_dummy_and_no_movement(policy):
     # --movementPolicy: locations, occupancy, sums and every statistic
     args = _cfg(n=2500, nl=400, weeks=1, seed=13, k="0.05", extra=["--movementPolicy", policy])
     run_parity(args, [1, 30, 144, 200, 1008], countdown=False)
+
+
+@pytest.mark.parametrize("n,infected", [(1, 0.3), (7, 0.3), (63, 0.3), (65, 0.3), (2049, 0.3), (8193, 0.3),
+                                        (2049, 0.0), (2049, 1.0)])
+def test_parity_tiny_ragged_and_extreme_populations(n, infected):
+    # populations smaller than one wave or one tile, and one agent past a
+    # movement tile (8192) or an infection tile (2048), so that the padded
+    # and partly filled tiles are exercised; nobody infected (no infection
+    # pass at all) and everybody infected at the start
+    args = pansim_amd.synthetic_args(n, 1, 3, 5, extra=["-k", "0.3", "--synthInfected", str(infected)])
+    run_parity(args, [1, 70, 144, 145, 300])
