@@ -1,8 +1,9 @@
 """The size-independent properties of tests/size_checks.py on the HIP path at
 the full size of the throughput measurement: the 10M-agent synthetic
 household population (4,603,331 locations, all policies on), after a
-daytime step of day 1, one step after the second midnight block, and at the
-activity change of 17:50 on day 2 (3.3M agents re-evaluated).  The oracle
+daytime step of day 1, one step after the second midnight block, at the
+activity change of 17:50 on day 2 (3.3M agents re-evaluated), one step
+after the last midnight block of the four weeks and at their last step.  The oracle
 does not run at this size here: a small oracle population built from the
 same input files supplies the state table and the variant multipliers, and
 the agents' possible-location lists are the ones held on the device."""
@@ -48,7 +49,7 @@ def full():
     sim.close()
 
 
-@pytest.mark.parametrize("target", [144 + 61, 2 * 144 + 1, 2 * 144 + 107])
+@pytest.mark.parametrize("target", [144 + 61, 2 * 144 + 1, 2 * 144 + 107, 27 * 144 + 1, 28 * 144 - 1])
 def test_full_size_invariants(full, target):
     sim, t = full
     assert sim.timestamp < target
