@@ -1,6 +1,6 @@
 """Build recipe of the native parts (no cmake): the gfx950 engine + host
-library (libpansim.so), the panSim CLI, and the CPU oracle used by the tests
-(oracle/_build/liboracle.so).
+library (libpansim.so), the panSim CLI, the pybind11 module pyPanSim, and the
+CPU oracle used by the tests (oracle/_build/liboracle.so).
 
 Compilers: hipcc for .hip (gfx950 code objects), g++ for plain C++.  Every
 translation unit is compiled with -ffp-contract=off so that host and device
