@@ -372,10 +372,13 @@ public:
         return days;
     }
     std::pair<unsigned, bool> selectNext(unsigned prog, unsigned state, float scaling, Rng& rng) {
+        return selectNextFor(prog, state, scaling, rng.unit());
+    }
+    // NextStates::selectNext for the uniform draw `random`
+    std::pair<unsigned, bool> selectNextFor(unsigned prog, unsigned state, float scaling, double random) const {
         const auto& bad = matrices[prog].bad[state];
         const auto& neu = matrices[prog].neutral[state];
         if (neu.empty()) scaling = 1.0f;
-        double random = rng.unit();
         double preSum = 0.0, badSum = 0.0;
         for (size_t i = 0; i < bad.size(); ++i) {
             float term = bad[i].second * scaling;
@@ -2353,6 +2356,16 @@ void oracle_state_table(void* h, unsigned char* wb, float* infectious) {
         infectious[k] = s->infTable[k];
     }
 }
+// The next state NextStates::selectNext picks for progression `prog`, state
+// `state`, symptom scaling `scaling` and uniform draw `u`; bit 16 set for a
+// bad progression.  -1 for indices out of range.
+int oracle_select_next(void* h, unsigned prog, unsigned state, float scaling, double u) {
+    auto* s = (oracle::Sim*)h;
+    if (prog >= s->matrices.size() || state >= s->matrices[prog].bad.size()) return -1;
+    auto r = s->selectNextFor(prog, state, scaling, u);
+    return (int)(r.first | (r.second ? 0x10000u : 0u));
+}
+unsigned oracle_num_progressions(void* h) { return (unsigned)((oracle::Sim*)h)->matrices.size(); }
 unsigned oracle_variant_multipliers(void* h, float* out, unsigned cap) {
     auto* s = (oracle::Sim*)h;
     for (unsigned v = 0; v < s->nVar && v < cap; ++v) out[v] = s->variantMul[v];

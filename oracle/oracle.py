@@ -54,6 +54,10 @@ def lib():
     L.oracle_agent_infectious.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
     L.oracle_variant_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
     L.oracle_variant_multipliers.restype = ctypes.c_uint32
+    L.oracle_select_next.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_double]
+    L.oracle_select_next.restype = ctypes.c_int
+    L.oracle_num_progressions.argtypes = [ctypes.c_void_p]
+    L.oracle_num_progressions.restype = ctypes.c_uint32
     dp = ctypes.POINTER(ctypes.c_double)
     L.oracle_set_type_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
     L.oracle_hazard_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -208,6 +212,17 @@ class OracleSim:
     def state_infectiousness_values(self) -> np.ndarray:
         """The infectiousness of every state (variant multiplier 1)."""
         return self.state_table()[1]
+
+    def select_next(self, prog: int, state: int, scaling: float, u: float):
+        """(next state, bad progression) of NextStates::selectNext for draw u."""
+        r = lib().oracle_select_next(self._h, prog, state, scaling, u)
+        if r < 0:
+            raise IndexError("progression or state out of range")
+        return r & 0xFFFF, bool(r >> 16)
+
+    @property
+    def num_progressions(self) -> int:
+        return lib().oracle_num_progressions(self._h)
 
     def variant_multipliers(self) -> np.ndarray:
         """The infectiousness multiplier of every variant."""
