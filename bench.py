@@ -28,7 +28,7 @@ same kernels from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 
 cpu_baseline: the oracle (sequential C++ re-statement of the reference
 per-step pipeline) on a bounded sample of the same workload (1M synthetic
-agents, one simulated day), on rank 0 only.
+agents, one simulated week: 1008 steps), on rank 0 only.
 """
 from __future__ import annotations
 
