@@ -1018,10 +1018,14 @@ int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_
     if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     if (v == 0) {
-        // the LocDyn records copied whole, the sums picked out on the host
-        std::vector<LocDyn> tmp(c->L.n);
-        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), c->L.dyn, tmp.size() * sizeof(LocDyn), hipMemcpyDeviceToHost));
-        for (uint32_t l = 0; l < c->L.n; ++l) out[l] = tmp[l].sum0;
+        // the LocDyn records copied whole in chunks, the sums picked out on the host
+        const uint32_t chunk = 1u << 20;
+        std::vector<LocDyn> tmp(std::min(c->L.n, chunk));
+        for (uint32_t first = 0; first < c->L.n; first += chunk) {
+            const uint32_t m = std::min(chunk, c->L.n - first);
+            HIP_OR_FAIL(c, hipMemcpy(tmp.data(), c->L.dyn + first, (size_t)m * sizeof(LocDyn), hipMemcpyDeviceToHost));
+            for (uint32_t l = 0; l < m; ++l) out[first + l] = tmp[l].sum0;
+        }
     } else
         HIP_OR_FAIL(c, hipMemcpy(out, c->L.sumX + (size_t)(v - 1) * c->L.n, c->L.n * sizeof(int64_t), hipMemcpyDeviceToHost));
     return 0;
@@ -1045,21 +1049,27 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
         for (uint32_t i = 0; i < cnt; ++i) out[i] = isSigned ? (uint32_t)(int32_t)(int8_t)tmp[i] : tmp[i];
         return 0;
     };
-    // one member of an array of records: the records are copied whole (one
-    // contiguous transfer; a strided 2D copy of millions of rows is slow)
-    // and the member is picked out on the host
+    // one member of an array of records: the records are copied whole, in
+    // chunks of at most 2^20 (contiguous transfers; a strided 2D copy of
+    // millions of rows is slow), and the member is picked out on the host
     auto getMember = [&](const void* base, size_t pitch, size_t offset, size_t width, uint32_t cnt) -> int {
         if (n < cnt) return fail(c, PANSIM_E_ARG, "output buffer too small");
-        std::vector<uint8_t> tmp((size_t)cnt * pitch);
-        HIP_OR_FAIL(c, hipMemcpy(tmp.data(), base, tmp.size(), hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint8_t* p = &tmp[(size_t)i * pitch + offset];
-            if (width == 4) memcpy(&out[i], p, 4);
-            else if (width == 2) {
-                uint16_t v;
-                memcpy(&v, p, 2);
-                out[i] = v;
-            } else out[i] = *p;
+        const uint32_t chunk = 1u << 20;
+        std::vector<uint8_t> tmp((size_t)std::min(cnt, chunk) * pitch);
+        for (uint32_t first = 0; first < cnt; first += chunk) {
+            const uint32_t m = std::min(chunk, cnt - first);
+            HIP_OR_FAIL(c, hipMemcpy(tmp.data(), (const uint8_t*)base + (size_t)first * pitch, (size_t)m * pitch,
+                                     hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint8_t* p = &tmp[(size_t)i * pitch + offset];
+                uint32_t& o = out[first + i];
+                if (width == 4) memcpy(&o, p, 4);
+                else if (width == 2) {
+                    uint16_t v;
+                    memcpy(&v, p, 2);
+                    o = v;
+                } else o = *p;
+            }
         }
         return 0;
     };
