@@ -38,11 +38,12 @@
 // Philox known-answer vectors, exp/log within one ulp of correctly rounded
 // values, the truncated-geometric mean, the geometric distribution, the
 // next-state shares of selectNext against the progression files' chances,
-// seasonality by date properties and exact cosine points, and the
-// invariants of movement, location sums and statistics.  Parity unpinned
-// (no reference test, golden vector or closed form covers them; they are
-// checked only as the GPU path's oracle): DetailedTesting, the closure
-// rules, otherDisease and the immunization waning pass.
+// seasonality by date properties and exact cosine points, the testing
+// pass one probability term at a time, and the invariants of movement,
+// location sums and statistics.  Parity unpinned (no reference test, golden
+// vector or closed form covers them; they are checked only as the GPU
+// path's oracle): the closure rules, otherDisease and the immunization
+// waning pass.
 #include <stdint.h>
 
 #include <algorithm>
