@@ -39,11 +39,12 @@
 // values, the truncated-geometric mean, the geometric distribution, the
 // next-state shares of selectNext against the progression files' chances,
 // seasonality by date properties and exact cosine points, the testing
-// pass one probability term at a time, and the invariants of movement,
-// location sums and statistics.  Parity unpinned (no reference test, golden
-// vector or closed form covers them; they are checked only as the GPU
-// path's oracle): the closure rules, otherDisease and the immunization
-// waning pass.
+// pass one probability term at a time, the first-dose protection rule of
+// the immunization pass, and the invariants of movement, location sums and
+// statistics.  Parity unpinned (no reference test, golden vector or closed
+// form covers them; they are checked only as the GPU path's oracle): the
+// closure rules, otherDisease, and the immunization pass after an infection
+// or a booster.
 #include <stdint.h>
 
 #include <algorithm>
