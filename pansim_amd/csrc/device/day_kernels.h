@@ -904,12 +904,13 @@ __global__ void k_plan_locs(PlanArgs a, uint32_t close) {
 
 constexpr uint32_t NOWORK_FLAGGED = 0xFFFFFFFFu;
 
-// checkUnderageAtHome (movementPolicies.h:1056-1094)
-__global__ void k_plan_underage(PlanArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// checkUnderageAtHome (movementPolicies.h:1056-1094), over the list of the
+// agents aged 14 or less (`under`, n entries)
+__global__ void k_plan_underage(PlanArgs a, const uint32_t* under, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
-    if (i >= A.n) return;
-    if (A.rec[i].age > 14) return;
+    if (j >= n) return;
+    const uint32_t i = under[j];
     const pansim_location_types& lt = a.T.lt;
     bool flag = false;
     if (A.rec[i].flags & F_QUARANTINED) flag = true;
@@ -929,22 +930,23 @@ __global__ void k_plan_underage(PlanArgs a) {
 }
 
 // setNoWorkToday (movementPolicies.h:1137-1158): the lowest eligible agent of
-// a flagged home claims it (claim pass, then resolve pass)
-__global__ void k_plan_nowork_claim(PlanArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// a flagged home claims it (claim pass, then resolve pass), over the list of
+// the agents aged 27-64 (`adults`, n entries)
+__global__ void k_plan_nowork_claim(PlanArgs a, const uint32_t* adults, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
-    if (i >= A.n) return;
-    if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
+    if (j >= n) return;
+    const uint32_t i = adults[j];
     uint32_t h = A.rec[i].home;
     // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent + 1)
     if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, i + 1u);
 }
 
-__global__ void k_plan_nowork_resolve(PlanArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_plan_nowork_resolve(PlanArgs a, const uint32_t* adults, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const Agents& A = a.A;
-    if (i >= A.n) return;
-    if (!(A.rec[i].age > 26 && A.rec[i].age < 65)) return;
+    if (j >= n) return;
+    const uint32_t i = adults[j];
     uint32_t h = A.rec[i].home;
     if (h != NOLOC && a.L.rec[h].noWork == i + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
 }

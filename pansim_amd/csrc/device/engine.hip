@@ -57,6 +57,11 @@ struct pansim_ctx {
     uint64_t stateEpoch = 1, varEpoch = 0;
     uint32_t varCache[MAXV] = {0};
     uint32_t* dummyWait = nullptr;  // DummyMovement countdowns (allocated on first use)
+    // agents by age band for the planning passes (ages do not change): under
+    // 15 (checkUnderageAtHome) and 27-64 (setNoWorkToday)
+    uint32_t* underageList = nullptr;
+    uint32_t* workAgeList = nullptr;
+    uint32_t nUnderage = 0, nWorkAge = 0;
     // infectious-list dump: the agents infected since the last collection,
     // and the collection's buffers (allocated when the dump is enabled)
     bool dumpOn = false;
@@ -381,6 +386,15 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
         }
         c->hcwCount = hcw;
         TRY(upload(c, &A.rec, ar.data(), N));
+        std::vector<uint32_t> under, work;
+        for (uint32_t k = 0; k < N; ++k) {
+            if (a->age[k] <= 14) under.push_back(k);
+            if (a->age[k] > 26 && a->age[k] < 65) work.push_back(k);
+        }
+        c->nUnderage = (uint32_t)under.size();
+        c->nWorkAge = (uint32_t)work.size();
+        if (!under.empty()) TRY(upload(c, &c->underageList, under.data(), under.size()));
+        if (!work.empty()) TRY(upload(c, &c->workAgeList, work.data(), work.size()));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
     TRY(upload(c, &A.prog, a->progression, N));
@@ -676,9 +690,15 @@ int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
     const uint32_t qp2 = p->quarantine_policy > 2 ? 1u : 0u;
     hipLaunchKernelGGL(k_plan_agents, dim3(gN), dim3(256), 0, c->stream, a, qp2);
     hipLaunchKernelGGL(k_plan_locs, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, a, qp2);
-    hipLaunchKernelGGL(k_plan_underage, dim3(gN), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_plan_nowork_claim, dim3(gN), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_plan_nowork_resolve, dim3(gN), dim3(256), 0, c->stream, a);
+    if (c->nUnderage)
+        hipLaunchKernelGGL(k_plan_underage, dim3(blocks_for(c->nUnderage)), dim3(256), 0, c->stream, a,
+                           (const uint32_t*)c->underageList, c->nUnderage);
+    if (c->nWorkAge) {
+        hipLaunchKernelGGL(k_plan_nowork_claim, dim3(blocks_for(c->nWorkAge)), dim3(256), 0, c->stream, a,
+                           (const uint32_t*)c->workAgeList, c->nWorkAge);
+        hipLaunchKernelGGL(k_plan_nowork_resolve, dim3(blocks_for(c->nWorkAge)), dim3(256), 0, c->stream, a,
+                           (const uint32_t*)c->workAgeList, c->nWorkAge);
+    }
     int r = check_launch(c, "plan_locations");
     if (r) return r;
     return rebuild_locations(c);
