@@ -40,11 +40,12 @@
 // next-state shares of selectNext against the progression files' chances,
 // seasonality by date properties and exact cosine points, the testing
 // pass one probability term at a time, the first-dose protection rule of
-// the immunization pass, and the invariants of movement, location sums and
-// statistics.  Parity unpinned (no reference test, golden vector or closed
-// form covers them; they are checked only as the GPU path's oracle): the
-// closure rules, otherDisease, and the immunization pass after an infection
-// or a booster.
+// the immunization pass, the afterDays closure window, and the invariants of
+// movement, location sums and statistics.  Parity unpinned (no reference
+// test, golden vector or closed form covers them; they are checked only as
+// the GPU path's oracle): the closure conditions on statistics and the
+// non-closing rule actions, otherDisease, and the immunization pass after an
+// infection or a booster.
 #include <stdint.h>
 
 #include <algorithm>
@@ -2321,6 +2322,7 @@ int oracle_field(void* h, const char* name, unsigned* out) {
     else if (n == "locQuarantineUntil") for (unsigned l = 0; l < L; ++l) out[l] = s->quarantineUntil[l];
     else if (n == "locClosedUntil") for (unsigned l = 0; l < L; ++l) out[l] = s->closedUntil[l];
     else if (n == "locOpen") for (unsigned l = 0; l < L; ++l) out[l] = s->locOpen[l];
+    else if (n == "locEssential") for (unsigned l = 0; l < L; ++l) out[l] = s->essential[l];
     else if (n == "locInfectiousness") for (unsigned l = 0; l < L; ++l) out[l] = fbits(s->locInf[l]);
     else if (n == "locCount") {
         s->rebuildLocationSums();

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 DATA_DIR = os.path.join(os.path.dirname(ORACLE_DIR), "pansim_amd", "data")
 
 _lib = None
-LOCATION_FIELDS = ["locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness", "locCount"]
+LOCATION_FIELDS = ["locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness", "locCount", "locEssential"]
 
 
 def lib():
