@@ -1,12 +1,19 @@
 """The oracle's closure rules (RuleClosure, closurePolicies.h:84-480) pinned
-for the "afterDays" condition by what its definition fixes
+by what the conditions' definitions fix.  "afterDays"
 (closurePolicies.h:117-125): a rule with threshold t and openAfter n is
 active on the days d (counted from 0 at the first midnight block) with
 t <= d < t + n, and while it is active every location of the listed types
 is closed, except the essential ones (closurePolicies.h:425-470), and only
 those.  Checked one step after each midnight block over
 twelve days, for rules with different windows and type lists, from rule
-files written here."""
+files written here.  "hospitalizedFraction" (closurePolicies.h:127-165): the
+day's fraction of agents in the hospital states (statistics entries whose
+state name contains "_h") enters a history of max(closeAfter, openAfter) + 2
+days; an inactive rule turns active when the fraction exceeds threshold and
+none of the last closeAfter values was below it, an active one turns
+inactive when the fraction is at most threshold2 and none of the last
+openAfter values reached it; the expected activity is derived from the
+run's own statistics vectors and compared with the closed locations."""
 import json
 
 import numpy as np
@@ -40,5 +47,42 @@ def test_after_days_rule_closes_exactly_its_types_in_its_window(tmp_path, thresh
                 assert np.array_equal(closed, listed), (day, int(closed.sum()), int(listed.sum()))
             else:
                 assert not closed.any(), (day, np.unique(lt[closed]))
+    finally:
+        o.close()
+
+
+def test_hospitalized_fraction_rule_follows_the_statistics(tmp_path):
+    thr, thr2, close_after, open_after, types = 0.02, 0.015, 2, 3, [5, 6]
+    rules = {"rules": [{"name": "hosp", "conditionType": "hospitalizedFraction", "threshold": thr,
+                        "threshold2": thr2, "openAfter": open_after, "closeAfter": close_after, "parameter": 0,
+                        "locationTypesToClose": types}]}
+    path = tmp_path / "rules.json"
+    path.write_text(json.dumps(rules))
+    args = pansim_amd.synthetic_args(5000, 6, 3, 2, extra=[
+        "-k", "0.05", "--synthInfected", "0.05", "--closures", str(path), "--enableClosures", "1"])
+    o = OracleSim(pansim_amd.with_data_dir(args))
+    try:
+        lt = o.location_types()
+        listed = np.isin(lt, types) & (o.field("locEssential") != 1)
+        hpos = [i for i, n in enumerate(o.state_names()) if "_h" in n]
+        assert len(hpos) == 3
+        hist = [0.0] * (max(close_after, open_after) + 2)
+        pos, active, switches = 0, False, 0
+        for day in range(40):
+            o.steps(day * SPD + 1 - o.timestamp)
+            st = o.day_stats(day)
+            value = sum(st[i] for i in hpos) / float(o.n_agents)
+            hist[pos] = value
+            pos = (pos + 1) % len(hist)
+            last = lambda k: [hist[(pos - 1 - j) % len(hist)] for j in range(k)]  # noqa: E731
+            was = active
+            if not active and value > thr:
+                active = not any(v < thr for v in last(close_after))
+            elif active and value <= thr2:
+                active = any(v >= thr2 for v in last(open_after))
+            switches += was != active
+            closed = o.field("locOpen") == 0
+            assert np.array_equal(closed, listed if active else np.zeros_like(listed)), (day, value, active)
+        assert switches >= 2  # the rule closed and reopened within the run
     finally:
         o.close()
