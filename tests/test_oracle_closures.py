@@ -13,7 +13,9 @@ days; an inactive rule turns active when the fraction exceeds threshold and
 none of the last closeAfter values was below it, an active one turns
 inactive when the fraction is at most threshold2 and none of the last
 openAfter values reached it; the expected activity is derived from the
-run's own statistics vectors and compared with the closed locations."""
+run's own statistics vectors and compared with the closed locations; the
+same for "newDeadFraction" (closurePolicies.h:166-205), whose value is the
+day's new COVID deaths (state D1) as a fraction of the agents."""
 import json
 
 import numpy as np
@@ -51,9 +53,11 @@ def test_after_days_rule_closes_exactly_its_types_in_its_window(tmp_path, thresh
         o.close()
 
 
-def test_hospitalized_fraction_rule_follows_the_statistics(tmp_path):
-    thr, thr2, close_after, open_after, types = 0.02, 0.015, 2, 3, [5, 6]
-    rules = {"rules": [{"name": "hosp", "conditionType": "hospitalizedFraction", "threshold": thr,
+@pytest.mark.parametrize("kind,thr,thr2,close_after,open_after", [
+    ("hospitalizedFraction", 0.02, 0.015, 2, 3), ("newDeadFraction", 0.0008, 0.0004, 2, 2)])
+def test_statistics_condition_follows_the_statistics(tmp_path, kind, thr, thr2, close_after, open_after):
+    types = [5, 6]
+    rules = {"rules": [{"name": "cond", "conditionType": kind, "threshold": thr,
                         "threshold2": thr2, "openAfter": open_after, "closeAfter": close_after, "parameter": 0,
                         "locationTypesToClose": types}]}
     path = tmp_path / "rules.json"
@@ -64,14 +68,21 @@ def test_hospitalized_fraction_rule_follows_the_statistics(tmp_path):
     try:
         lt = o.location_types()
         listed = np.isin(lt, types) & (o.field("locEssential") != 1)
-        hpos = [i for i, n in enumerate(o.state_names()) if "_h" in n]
+        names = o.state_names()
+        hpos = [i for i, n in enumerate(names) if "_h" in n]
         assert len(hpos) == 3
+        dead = names.index("D1")
+        prev_dead = 0
         hist = [0.0] * (max(close_after, open_after) + 2)
         pos, active, switches = 0, False, 0
         for day in range(40):
             o.steps(day * SPD + 1 - o.timestamp)
             st = o.day_stats(day)
-            value = sum(st[i] for i in hpos) / float(o.n_agents)
+            if kind == "hospitalizedFraction":
+                value = sum(st[i] for i in hpos) / float(o.n_agents)
+            else:  # new COVID deaths since the previous midnight
+                value = (st[dead] - prev_dead) / float(o.n_agents)
+                prev_dead = st[dead]
             hist[pos] = value
             pos = (pos + 1) % len(hist)
             last = lambda k: [hist[(pos - 1 - j) % len(hist)] for j in range(k)]  # noqa: E731
