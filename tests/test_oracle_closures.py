@@ -97,3 +97,32 @@ def test_statistics_condition_follows_the_statistics(tmp_path, kind, thr, thr2, 
         assert switches >= 2  # the rule closed and reopened within the run
     finally:
         o.close()
+
+
+def test_masks_rule_scales_non_home_infectiousness_in_its_window(tmp_path):
+    # Masks (closurePolicies.h:214-245): while active, every non-home
+    # location's infectiousness is multiplied by the parameter (a float times
+    # a double, stored as float); when it ends, divided by it again
+    rules = {"rules": [{"name": "Masks", "conditionType": "afterDays", "threshold": 2, "threshold2": 2,
+                        "openAfter": 3, "closeAfter": -1, "parameter": 0.8, "locationTypesToClose": []}]}
+    path = tmp_path / "rules.json"
+    path.write_text(json.dumps(rules))
+    args = pansim_amd.synthetic_args(3000, 2, 3, 2, extra=["--closures", str(path), "--enableClosures", "1"])
+    o = OracleSim(pansim_amd.with_data_dir(args))
+    try:
+        home = int(json.load(open(pansim_amd.DATA_DIR + "/inputConfigFiles/locationTypes.json"))["home"])
+        non_home = o.location_types() != home
+        inf = lambda: o.field("locInfectiousness").view(np.float32).copy()  # noqa: E731
+        o.steps(1)
+        orig = inf()
+        assert (orig[non_home] > 0).any()
+        masked = orig.copy()
+        masked[non_home] = (orig[non_home].astype(np.float64) * 0.8).astype(np.float32)
+        after = masked.copy()
+        after[non_home] = (masked[non_home].astype(np.float64) / 0.8).astype(np.float32)
+        for day in range(8):
+            o.steps(day * SPD + 1 - o.timestamp)
+            expect = masked if 2 <= day < 5 else (after if day >= 5 else orig)
+            assert np.array_equal(inf(), expect), day
+    finally:
+        o.close()
