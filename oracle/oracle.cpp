@@ -40,13 +40,12 @@
 // next-state shares of selectNext against the progression files' chances,
 // seasonality by date properties and exact cosine points, the testing
 // pass one probability term at a time, the first-dose protection rule of
-// the immunization pass, the three closure conditions and the Masks
-// action, and the invariants
-// of movement, location sums and statistics.  Parity unpinned (no reference
-// test, golden vector or closed form covers them; they are checked only as
-// the GPU path's oracle): the non-closing rule actions other than Masks
-// (Curfew, HolidayMode, ExposeToMutation, ReduceMovement, ...), otherDisease,
-// and the immunization pass after an infection or a booster.
+// the immunization pass, the three closure conditions, the Masks and
+// ExposeToMutation actions, and the invariants of movement, location sums
+// and statistics.  Parity unpinned (no reference test, golden vector or
+// closed form covers them; they are checked only as the GPU path's oracle):
+// the Curfew, HolidayMode and ReduceMovement actions, otherDisease, and the
+// immunization pass after an infection or a booster.
 #include <stdint.h>
 
 #include <algorithm>

@@ -126,3 +126,33 @@ def test_masks_rule_scales_non_home_infectiousness_in_its_window(tmp_path):
             assert np.array_equal(inf(), expect), day
     finally:
         o.close()
+
+
+def test_expose_to_mutation_infects_the_given_share_of_the_susceptible(tmp_path):
+    # ExposeToMutation (closurePolicies.h:347-391): on the rule's day each
+    # susceptible agent gets the variant given by threshold2 with probability
+    # parameter * susceptibility (1 for every agent in S before any
+    # vaccination), so the exposed count is Binomial(S, parameter), every one
+    # of them in the first infected state with that variant
+    frac, variant, day = 0.05, 1, 3
+    rules = {"rules": [{"name": "ExposeToMutation", "conditionType": "afterDays", "threshold": day,
+                        "threshold2": variant, "openAfter": 1, "closeAfter": -1, "parameter": frac,
+                        "locationTypesToClose": []}]}
+    path = tmp_path / "rules.json"
+    path.write_text(json.dumps(rules))
+    args = pansim_amd.synthetic_args(20000, 2, 3, 2, extra=["--closures", str(path), "--enableClosures", "1"])
+    o = OracleSim(pansim_amd.with_data_dir(args))
+    try:
+        o.steps(day * SPD + 1)
+        ts = day * SPD
+        names = o.state_names()
+        susceptible = o.day_stats(day)[names.index("S")]  # taken before the rules ran
+        new = o.field("infectedTimestamp") == ts
+        var = o.field("variant")
+        exposed = new & (var == variant)
+        n = int(exposed.sum())
+        assert abs(n - frac * susceptible) <= 5.0 * np.sqrt(susceptible * frac * (1 - frac)), (n, susceptible)
+        assert n > 0 and np.all(o.field("state")[exposed] == names.index("E"))
+        assert not np.any((var == variant) & ~new)  # nobody else carries the variant
+    finally:
+        o.close()
