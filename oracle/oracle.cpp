@@ -40,12 +40,12 @@
 // next-state shares of selectNext against the progression files' chances,
 // seasonality by date properties and exact cosine points, the testing
 // pass one probability term at a time, the first-dose protection rule of
-// the immunization pass, the three closure conditions, the Masks and
-// ExposeToMutation actions, and the invariants of movement, location sums
-// and statistics.  Parity unpinned (no reference test, golden vector or
-// closed form covers them; they are checked only as the GPU path's oracle):
-// the Curfew, HolidayMode and ReduceMovement actions, otherDisease, and the
-// immunization pass after an infection or a booster.
+// the immunization pass, the three closure conditions, the Masks,
+// ExposeToMutation and Curfew actions, and the invariants of movement,
+// location sums and statistics.  Parity unpinned (no reference test, golden
+// vector or closed form covers them; they are checked only as the GPU
+// path's oracle): the HolidayMode and ReduceMovement actions, otherDisease,
+// and the immunization pass after an infection or a booster.
 #include <stdint.h>
 
 #include <algorithm>
@@ -2292,6 +2292,7 @@ int oracle_field(void* h, const char* name, unsigned* out) {
         return u;
     };
     if (n == "state") for (unsigned i = 0; i < N; ++i) out[i] = (unsigned)s->pp[i].state;
+    else if (n == "agentType") for (unsigned i = 0; i < N; ++i) out[i] = s->types[i];
     else if (n == "variant") for (unsigned i = 0; i < N; ++i) out[i] = s->pp[i].variant;
     else if (n == "days") for (unsigned i = 0; i < N; ++i) out[i] = (unsigned)(int)s->pp[i].days;
     else if (n == "location") for (unsigned i = 0; i < N; ++i) out[i] = s->location[i];

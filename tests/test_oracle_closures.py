@@ -156,3 +156,42 @@ def test_expose_to_mutation_infects_the_given_share_of_the_susceptible(tmp_path)
         assert not np.any((var == variant) & ~new)  # nobody else carries the variant
     finally:
         o.close()
+
+
+def test_curfew_sends_everyone_home_at_its_start(tmp_path):
+X
+    # movementPolicies.h:267-326): at the curfew's first step every living
+    # agent not in hospital goes home, except the agents of type 7 (night
+    # workers, who may stay at an open workplace); on a day without the
+    # curfew some agents are still out at that time
+    rules = {"rules": [{"name": "Curfew", "conditionType": "afterDays", "threshold": 1, "threshold2": 1,
+                        "openAfter": 3, "closeAfter": -1, "parameter": "2000-0500", "locationTypesToClose": []}]}
+    path = tmp_path / "rules.json"
+    path.write_text(json.dumps(rules))
+    args = pansim_amd.synthetic_args(5000, 2, 3, 2, extra=[
+        "-k", "0.05", "--synthInfected", "0.05", "--closures", str(path), "--enableClosures", "1"])
+    o = OracleSim(pansim_amd.with_data_dir(args))
+    try:
+        off, locs, types = o.csr()
+        n = o.n_agents
+        home_t = int(json.load(open(pansim_amd.DATA_DIR + "/inputConfigFiles/locationTypes.json"))["home"])
+        home = np.full(n, -1, dtype=np.int64)
+        owner = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+        is_home = types == home_t
+        first = np.unique(owner[is_home], return_index=True)
+        home[first[0]] = locs[is_home][first[1]]
+        wb, _ = o.state_table()
+        step_20 = 20 * 60 // 10
+        for day, curfew in ((1, True), (2, True), (3, True), (5, False)):
+            o.steps(day * SPD + step_20 + 1 - o.timestamp)
+            ts = day * SPD + step_20
+            w = wb[o.field("state").astype(np.int64)]
+            hosp = (o.field("hospitalizedTimestamp") <= ts) & (o.field("hospitalizedUntilTimestamp") > ts)
+            subject = (w != 3) & (w != 4) & ~hosp & (o.field("agentType") != 6) & (home >= 0)
+            away = subject & (o.field("location").astype(np.int64) != home)
+            if curfew:
+                assert not away.any(), (day, int(away.sum()))
+            else:
+                assert away.sum() > 0
+    finally:
+        o.close()
