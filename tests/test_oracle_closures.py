@@ -159,7 +159,7 @@ def test_expose_to_mutation_infects_the_given_share_of_the_susceptible(tmp_path)
 
 
 def test_curfew_sends_everyone_home_at_its_start(tmp_path):
-X
+    # Curfew (closurePolicies.h:247-272; doMovement's curfew branch,
     # movementPolicies.h:267-326): at the curfew's first step every living
     # agent not in hospital goes home, except the agents of type 7 (night
     # workers, who may stay at an open workplace); on a day without the
