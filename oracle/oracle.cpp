@@ -41,10 +41,11 @@
 // seasonality by date properties and exact cosine points, the testing
 // pass one probability term at a time, the first-dose protection rule of
 // the immunization pass, the three closure conditions, the Masks,
-// ExposeToMutation and Curfew actions, and the invariants of movement,
-// location sums and statistics.  Parity unpinned (no reference test, golden
-// vector or closed form covers them; they are checked only as the GPU
-// path's oracle): the HolidayMode and ReduceMovement actions, otherDisease,
+// ExposeToMutation and Curfew actions, the structure of the non-COVID
+// hospital stays, and the invariants of movement, location sums and
+// statistics.  Parity unpinned (no reference test, golden vector or closed
+// form covers them; they are checked only as the GPU path's oracle): the
+// HolidayMode and ReduceMovement actions, otherDisease's probability table,
 // and the immunization pass after an infection or a booster.
 #include <stdint.h>
 
