@@ -67,6 +67,18 @@
 
 namespace oracle {
 
+// Planted mistakes, for the tests that show a pin fails on a wrong oracle
+// (0 = none; set through oracle_plant).  Each code breaks one rule:
+//  1 Case 3 waits up to 60 min instead of 30 before moving (:883)
+//  2 a single event's stay is one step longer than its duration (:796)
+//  3 the cumulative event pick compares with >= (:666-672)
+//  4 a closed destination is not noticed (:706-709)
+//  5 a location-quarantined agent is not sent home (:964-975)
+//  6 a diagnosed agent's work/classroom is quarantined only above policy 3 (:209)
+//  7 the last eligible adult of a home takes the child care (:1149-1155)
+//  8 one quarantined classroom already quarantines the school (:1297)
+int plant = 0;
+
 using ps::Scenario;
 constexpr unsigned UMAX = 0xFFFFFFFFu;
 constexpr int MAXS = 7;
@@ -650,7 +662,7 @@ void Sim::quarantineLocations(unsigned i, unsigned until, unsigned previousQuara
                               std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng) {
     unsigned myHome = findLoc(i, homeT, 0, false, nullptr);
     if (quarantinePolicy > 1 && myHome != UMAX) writes.emplace_back(myHome, until);  // the reference writes out of bounds
-    if (quarantinePolicy > 2 && previousQuarantineUntil < timestamp) {
+    if (quarantinePolicy > (plant == 6 ? 3u : 2u) && previousQuarantineUntil < timestamp) {
         unsigned classroom = findLoc(i, classroomT, 0, false, &rng);
         unsigned work = findLoc(i, workT, 0, false, nullptr);
         unsigned toClose[2] = {classroom, work};
@@ -751,7 +763,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
         if (quarantined[i]) {
             sum = umin(locQ(location[i]) - timestamp - 1, stepsUntilMidnight);
             unsigned homeLocation = findLoc(i, homeT, 0, false, nullptr);
-            if (homeLocation != location[i]) location[i] = homeLocation;
+            if (homeLocation != location[i] && plant != 5) location[i] = homeLocation;
             if (agentHome != UMAX && location[i] != agentHome && sum > 60 / dt) stayedHome[i] = 0;
             return;
         }
@@ -800,7 +812,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
             double rand = rng.real(1.0);
             double thr = events[activeBegin].chance;
             unsigned idx = 0;
-            while (rand > thr && idx < numPotential) {
+            while ((plant == 3 ? rand >= thr - 0.1 : rand > thr) && idx < numPotential) {
                 idx++;
                 if (idx < numPotential) thr += events[activeBegin + idx].chance;  // the reference also reads one past
             }
@@ -818,6 +830,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
         bool schoolAndTooOld =
             (newLocationType == schoolT || newLocationType == classroomT) && meta[i].age >= schoolAgeRestriction;
         auto closedHere = [&](unsigned loc, unsigned lt) {
+            if (plant == 4) return false;
             return (openAt(loc) == 0 || closedAt(loc) > timestamp) && lt != workT;
         };
         if (schoolAndTooOld || closedHere(newLocation, newLocationType) ||
@@ -857,7 +870,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
             // event of the next schedule slot (a default event past the array end)
             const Event ne = (unsigned)activeEnd < events.size() ? events[activeEnd] : Event{0, -1.0f, TD(), TD(), TD()};
             TD arrive = simTime.plus(basicDuration);
-            if (arrive < ne.start) sum = basicDuration.steps(dt);
+            if (arrive < ne.start) sum = basicDuration.steps(dt) + (plant == 2 ? 1u : 0u);
             else if (arrive > ne.end) sum = ne.end.minus(simTime).steps(dt) - 1;
             else {
                 TD window = ne.end.minus(arrive);
@@ -881,7 +894,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
         unsigned randDelay = rng.below(lengthSteps);
         sum = ne.start.minus(simTime).steps(dt) + randDelay;
         unsigned timeLeft = sum;
-        if (timeLeft < 30 / dt) {
+        if (timeLeft < (plant == 1 ? 60u : 30u) / dt) {
             // stay
         } else if (timeLeft < 60 / dt) {
             newLocationType = publicT;
@@ -897,7 +910,7 @@ void Sim::doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng) {
         if (quarantined[i]) {
             sum = umin(locQ(location[i]) - timestamp - 1, stepsUntilMidnight);
             unsigned homeLocation = findLoc(i, homeT, 0, false, nullptr);
-            if (homeLocation != location[i]) location[i] = homeLocation;
+            if (homeLocation != location[i] && plant != 5) location[i] = homeLocation;
             if (agentHome != UMAX && location[i] != agentHome && sum > 60 / dt) stayedHome[i] = 0;
             return;
         }
@@ -1071,7 +1084,7 @@ void Sim::planLocations() {
                     mx = mx > quarantineUntil[cl] ? mx : quarantineUntil[cl];
                 }
             }
-            if (counter > 1) {
+            if (counter > (plant == 8 ? 0u : 1u)) {
                 quarantineUntil[schoolIdx] = mx;
                 for (unsigned c = classroomOffsets[s]; c < classroomOffsets[s + 1]; ++c) quarantineUntil[classrooms[c]] = mx;
             }
@@ -1122,7 +1135,8 @@ void Sim::planLocations() {
             }
         }
     }
-    for (unsigned i = 0; i < N; ++i) {  // setNoWorkToday, ascending agent order
+    for (unsigned k = 0; k < N; ++k) {  // setNoWorkToday, ascending agent order
+        const unsigned i = plant == 7 ? N - 1 - k : k;
         if (meta[i].age > 26 && meta[i].age < 65) {
             uint32_t b = locOffset[i], e = locOffset[i + 1];
             unsigned h = UMAX;
@@ -2220,6 +2234,8 @@ void* oracle_create(int argc, const char** argv) {
 }
 
 const char* oracle_last_error() { return g_err.c_str(); }
+
+void oracle_plant(int code) { oracle::plant = code; }
 
 void oracle_destroy(void* h) { delete (oracle::Sim*)h; }
 

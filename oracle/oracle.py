@@ -74,6 +74,7 @@ def lib():
     L.oracle_length_p.restype = ctypes.c_double
     L.oracle_geometric.argtypes = [ctypes.c_ulonglong, ctypes.c_uint32, ctypes.c_double]
     L.oracle_geometric.restype = ctypes.c_int
+    L.oracle_plant.argtypes = [ctypes.c_int]
     _lib = L
     return L
 
@@ -238,6 +239,12 @@ class OracleSim:
 
 def _u32(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def plant(code: int) -> None:
+    """Switches one planted mistake on in the oracle (0 switches it off);
+    for tests that show a check fails on a wrong oracle."""
+    lib().oracle_plant(code)
 
 
 def philox(ctr, key):
