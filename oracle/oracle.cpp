@@ -318,7 +318,7 @@ public:
     void initDays();
 
     // reference routines
-    unsigned findLoc(unsigned agent, unsigned locType, int retry, bool useStates, Rng* rng);
+    unsigned findLoc(unsigned agent, unsigned locType, int retry, bool useStates, Rng* rng, bool plan = false);
     void quarantineAgent(unsigned i, unsigned until);
     void quarantineLocations(unsigned i, unsigned until, unsigned previousQuarantineUntil,
                              std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng);
@@ -591,9 +591,13 @@ void Sim::initDays() {
 }
 
 // movementPolicies.h:69-122
-unsigned Sim::findLoc(unsigned agent, unsigned lt, int retry, bool useStates, Rng* rng) {
+// movementPolicies.h:69-122.  The reference's callers name three types whose
+// first entry is taken without a draw: home, school and work, except the
+// planning helpers checkUnderageAtHome and checkSchoolWorkQuarantine
+// (:1073-1090, :1208-1213), which name home, school and classroom (`plan`).
+unsigned Sim::findLoc(unsigned agent, unsigned lt, int retry, bool useStates, Rng* rng, bool plan) {
     uint32_t b = locOffset[agent], e = locOffset[agent + 1];
-    if (lt == homeT || lt == schoolT || lt == workT) {
+    if (lt == homeT || lt == schoolT || lt == (plan ? classroomT : workT)) {
         for (uint32_t i = b; i < e; ++i)
             if (lt == possibleTypes[i]) return possibleLocations[i];
         return UMAX;
@@ -1095,9 +1099,9 @@ void Sim::planLocations() {
             if (quarantined[i]) continue;
             if (stats[i].diagnosedTimestamp > 0 && !isInfected(i)) continue;
             Rng rng(seed, i, timestamp, PHASE_PLAN_SCHOOLWORK);
-            unsigned wl = findLoc(i, workT, 0, false, nullptr);
-            unsigned sl = findLoc(i, schoolT, 0, false, nullptr);
-            unsigned cl = findLoc(i, classroomT, 0, false, &rng);
+            unsigned wl = findLoc(i, workT, 0, false, &rng, true);
+            unsigned sl = findLoc(i, schoolT, 0, false, nullptr, true);
+            unsigned cl = findLoc(i, classroomT, 0, false, nullptr, true);
             unsigned until = 0;
             if (sl != UMAX && quarantineUntil[sl] > timestamp) until = quarantineUntil[sl];
             else if (cl != UMAX && quarantineUntil[cl] > timestamp) until = quarantineUntil[cl];
@@ -1124,9 +1128,8 @@ void Sim::planLocations() {
             unsigned h = findLoc(i, homeT, 0, false, nullptr);
             if (h != UMAX) noWorkLoc[h] = 1;
         } else {
-            Rng rng(seed, i, timestamp, PHASE_PLAN_UNDERAGE);
-            unsigned sl = findLoc(i, schoolT, 0, false, nullptr);
-            unsigned cl = findLoc(i, classroomT, 0, false, &rng);
+            unsigned sl = findLoc(i, schoolT, 0, false, nullptr, true);
+            unsigned cl = findLoc(i, classroomT, 0, false, nullptr, true);
             bool tooOld = meta[i].age >= schoolAgeRestriction;
             if (tooOld || (sl != UMAX && (locOpen[sl] == 0 || closedUntil[sl] > timestamp)) ||
                 (cl != UMAX && (locOpen[cl] == 0 || closedUntil[cl] > timestamp))) {

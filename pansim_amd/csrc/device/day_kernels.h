@@ -849,9 +849,11 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
         const pansim_location_types& lt = a.T.lt;
         Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
         const CsrReg csr = load_csr(A, i, r);
-        uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
-        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
-        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
+        // the planning lookup (home, school, classroom first-match): a
+        // workplace is drawn when the agent has several, in this order
+        uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, &rs, true);
+        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr, true);
+        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, nullptr, true);
         // the three quarantine times loaded together, then the first that applies
         const LocRec* q = a.L.rec;
         const uint32_t qs = sl != NOLOC ? q[sl].quarUntil : 0u;
@@ -915,10 +917,9 @@ __global__ void k_plan_underage(PlanArgs a, const uint32_t* under, uint32_t n) {
     bool flag = false;
     if (A.rec[i].flags & F_QUARANTINED) flag = true;
     else {
-        Stream rs(a.seed, i, a.ts, PH_PLAN_UNDERAGE, a.replica);
         const CsrReg csr = load_csr(A, i, A.rec[i]);
-        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
-        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
+        uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr, true);
+        uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, nullptr, true);
         bool tooOld = A.rec[i].age >= a.schoolAge;
         flag = tooOld || (sl != NOLOC && (a.L.rec[sl].open == 0 || a.L.rec[sl].closedUntil > a.ts)) ||
                (cl != NOLOC && (a.L.rec[cl].open == 0 || a.L.rec[cl].closedUntil > a.ts));

@@ -99,11 +99,22 @@ struct Csr {
 __device__ __forceinline__ Csr csr_of(const AgentRec& r) { return Csr{r.csrBegin, r.csrBegin + r.csrCount, r.home}; }
 
 // movementPolicies.h:69-122; `rs` may be null when the type needs no draw.
+// The reference's lookup takes the first entry, without a draw, for three
+// types passed by the caller: home, school and work everywhere except in the
+// planning helpers (checkUnderageAtHome, checkSchoolWorkQuarantine,
+// movementPolicies.h:1073-1090, 1208-1213), which pass home, school and
+// classroom: there a classroom is the first entry of its type and a
+// workplace is drawn among the agent's workplaces.  `plan` selects the
+// second set.
+__device__ __forceinline__ bool first_match_type(const pansim_location_types& lt, uint32_t ty, bool plan) {
+    return ty == lt.school || ty == (plan ? lt.classroom : lt.work);
+}
+
 __device__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt, Csr c, uint32_t ty,
-                             int retry, Stream* rs) {
+                             int retry, Stream* rs, bool plan = false) {
     if (ty == lt.home) return c.home;
     const uint32_t b = c.b, e = c.e;
-    if (ty == lt.school || ty == lt.work) {
+    if (first_match_type(lt, ty, plan)) {
         for (uint32_t i = b; i < e; ++i)
             if (A.possType[i] == ty) return A.possLoc[i];
         return NOLOC;
@@ -199,13 +210,13 @@ __device__ __forceinline__ uint32_t nth_bit(uint32_t mask, uint32_t r) {
 // findActualLocationForType over the register copy of the types; lists longer
 // than CSR_REG_MAX take the memory version.
 __device__ __forceinline__ uint32_t find_loc(const Agents& A, const Locs& L, const pansim_location_types& lt,
-                                             const CsrReg& c, uint32_t ty, int retry, Stream* rs) {
+                                             const CsrReg& c, uint32_t ty, int retry, Stream* rs, bool plan = false) {
     if (ty == lt.home) return c.home;
-    if (c.n > CSR_REG_MAX) return find_loc(A, L, lt, Csr{c.b, c.b + c.n, c.home}, ty, retry, rs);
+    if (c.n > CSR_REG_MAX) return find_loc(A, L, lt, Csr{c.b, c.b + c.n, c.home}, ty, retry, rs, plan);
     const uint32_t mask = ty <= 0xFFu ? csr_match(c, ty) : 0u;  // list types are bytes
     if (mask == 0u) return NOLOC;
-    // school and work: the first entry of the type, no draw
-    if (ty == lt.school || ty == lt.work || (mask & (mask - 1u)) == 0u)
+    // the caller's first-match types, or a single entry: no draw
+    if (first_match_type(lt, ty, plan) || (mask & (mask - 1u)) == 0u)
         return A.possLoc[c.b + (uint32_t)__builtin_ctz(mask)];
     const uint32_t counter = (uint32_t)__builtin_popcount(mask);
     uint32_t counter2 = counter, randIdx = counter;

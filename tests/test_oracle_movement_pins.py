@@ -444,3 +444,37 @@ def test_pins_fail_on_a_planted_mistake(code, check):
             check()
     finally:
         plant(0)
+
+
+def test_planning_lookup_draws_a_workplace_and_takes_the_first_classroom():
+    """The planning helpers look locations up with home, school and classroom
+    as the first-match types (:1208-1213): an agent with two workplaces has
+    one drawn at random, an agent with two classrooms gets the first.
+    A (I1) is diagnosed at hospital at 8:00 and, under policy 3, quarantines
+    its workplace W1 and its classroom C0 until 1488.  400 workers list W0
+    then W1, 400 pupils list C1 then C0 (agent types with weekend schedules
+    only: on this Monday they stay at home and meet nobody).  At the next
+    midnight about half of the workers draw W1 and are quarantined, and no
+    pupil is (its classroom is C1)."""
+    shops = [event(SHOP7, 1, 15, 15.1, 0.3), event(SHOP7, 1, 16, 16.1, 0.3)]
+    t_a = agent_type(1, [schedule("W", "ALL", [event(HOSPITAL, 1, 8, 8.1, 1)] + shops)])
+    t_weekend = agent_type(2, [schedule("W", "WEEKENDS", [event(SHOP7, 1, 10, 10.1, 1)])])
+    places = [place("SC0", 3), place("C0_SC0", 33), place("C1_SC0", 33), place("H0", HOME), place("H1", HOME),
+              place("W0", WORK), place("W1", WORK), place("S7", SHOP7), place("HOSP", HOSPITAL)]
+    n = 400
+    people = [person(1, [(HOME, "H0"), (WORK, "W1"), (33, "C0_SC0"), (SHOP7, "S7"), (HOSPITAL, "HOSP")],
+                     state="I1")]
+    people += [person(2, [(HOME, "H1"), (WORK, "W0"), (WORK, "W1")]) for _ in range(n)]
+    people += [person(2, [(HOME, "H1"), (3, "SC0"), (33, "C1_SC0"), (33, "C0_SC0")], age=10) for _ in range(n)]
+    w = MiniWorld(places, people, [t_a, t_weekend])
+    o = OracleSim(w.args(quarantine_policy=3))
+    try:
+        o.steps(145)
+        q = o.field("quarantined")
+        until = o.field("quarantinedUntilTimestamp")
+    finally:
+        o.close()
+    workers, pupils = q[1:n + 1], q[n + 1:]
+    assert abs(workers.mean() - 0.5) <= 4.5 * np.sqrt(0.25 / n)
+    assert set(until[1:n + 1][workers == 1].tolist()) == {1488}
+    assert pupils.sum() == 0
