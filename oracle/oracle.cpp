@@ -2313,6 +2313,9 @@ int oracle_field(void* h, const char* name, unsigned* out) {
     };
     if (n == "state") for (unsigned i = 0; i < N; ++i) out[i] = (unsigned)s->pp[i].state;
     else if (n == "agentType") for (unsigned i = 0; i < N; ++i) out[i] = s->types[i];
+    else if (n == "age") for (unsigned i = 0; i < N; ++i) out[i] = s->meta[i].age;
+    else if (n == "sex") for (unsigned i = 0; i < N; ++i) out[i] = s->meta[i].sex ? 1u : 0u;
+    else if (n == "preCond") for (unsigned i = 0; i < N; ++i) out[i] = s->meta[i].preCondIdx;
     else if (n == "variant") for (unsigned i = 0; i < N; ++i) out[i] = s->pp[i].variant;
     else if (n == "days") for (unsigned i = 0; i < N; ++i) out[i] = (unsigned)(int)s->pp[i].days;
     else if (n == "location") for (unsigned i = 0; i < N; ++i) out[i] = s->location[i];
@@ -2375,6 +2378,21 @@ void oracle_csr(void* h, unsigned* offsets, unsigned* locs, unsigned char* types
     }
 }
 unsigned oracle_cemetery(void* h) { return ((oracle::Sim*)h)->cemetery; }
+// The schedule tables: eventOffset (28 * types + 1 entries) and, per event,
+// its location type, chance and (start, end, duration) as (hours, minutes).
+unsigned oracle_num_events(void* h) { return (unsigned)((oracle::Sim*)h)->events.size(); }
+unsigned oracle_num_offsets(void* h) { return (unsigned)((oracle::Sim*)h)->eventOffset.size(); }
+void oracle_schedule(void* h, unsigned* offsets, unsigned* types, float* chances, unsigned char* times) {
+    auto* s = (oracle::Sim*)h;
+    for (size_t i = 0; i < s->eventOffset.size(); ++i) offsets[i] = s->eventOffset[i];
+    for (size_t i = 0; i < s->events.size(); ++i) {
+        const auto& e = s->events[i];
+        types[i] = e.locationType;
+        chances[i] = e.chance;
+        unsigned char t[6] = {e.start.h, e.start.m, e.end.h, e.end.m, e.duration.h, e.duration.m};
+        for (int k = 0; k < 6; ++k) times[6 * i + k] = t[k];
+    }
+}
 void oracle_location_types(void* h, unsigned* out) {
     auto* s = (oracle::Sim*)h;
     for (unsigned l = 0; l < s->L; ++l) out[l] = s->locType[l];

@@ -75,6 +75,11 @@ def lib():
     L.oracle_geometric.argtypes = [ctypes.c_ulonglong, ctypes.c_uint32, ctypes.c_double]
     L.oracle_geometric.restype = ctypes.c_int
     L.oracle_plant.argtypes = [ctypes.c_int]
+    for f in ("oracle_num_events", "oracle_num_offsets"):
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+        getattr(L, f).restype = ctypes.c_uint32
+    L.oracle_schedule.argtypes = [ctypes.c_void_p, u32p, u32p, ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_uint8)]
     _lib = L
     return L
 
@@ -188,6 +193,18 @@ class OracleSim:
         types = np.zeros(max(1, m), dtype=np.uint8)
         lib().oracle_csr(self._h, _u32(off), _u32(locs), types.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
         return off, locs[:m], types[:m]
+
+    def schedule(self):
+        """(eventOffset, location types, chances, times[n, 6]) of the parsed
+        agent-type schedules; times = start h, m, end h, m, duration h, m."""
+        ne, no = lib().oracle_num_events(self._h), lib().oracle_num_offsets(self._h)
+        off = np.zeros(no, dtype=np.uint32)
+        ty = np.zeros(max(1, ne), dtype=np.uint32)
+        ch = np.zeros(max(1, ne), dtype=np.float32)
+        tm = np.zeros((max(1, ne), 6), dtype=np.uint8)
+        lib().oracle_schedule(self._h, _u32(off), _u32(ty), ch.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                              tm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return off, ty[:ne], ch[:ne], tm[:ne]
 
     @property
     def cemetery(self) -> int:
