@@ -55,3 +55,86 @@ def test_stays_and_non_covid_deaths_follow_the_rules():
         assert starts > 200 and ends > 200 and deaths > 10
     finally:
         o.close()
+
+
+# the reference's admission probabilities (simulation.h:226-310) for the cells
+# checked below: (precondition id, sex, age) -> (admission, death at the end)
+CELLS = {
+    ("0", "M", 2): (0.00017785, 0.006333678), ("0", "M", 10): (0.000118567, 0.004222452),
+    ("0", "M", 20): (0.000614453, 0.008444904), ("0", "M", 45): (0.003174864, 0.030227275),
+    ("0", "M", 65): (0.001016573, 0.036202732), ("0", "M", 75): (0.0011019, 0.03924143),
+    ("0", "M", 85): (0.000926381, 0.032990764), ("0", "F", 45): (0.000986754, 0.009394699),
+    ("0", "F", 85): (0.000855563, 0.030468759), ("2", "M", 45): (0.006390329, 0.013129695),
+    ("2", "F", 75): (0.000653666, 0.009649877), ("4", "M", 45): (0.006378891, 0.058278861),
+    ("4", "F", 65): (0.000465264, 0.030884712), ("1", "M", 45): (0.003174864, 0.030227275),
+}
+
+
+def test_admission_rates_lengths_and_deaths_follow_the_table():
+    """Per cell of (precondition, sex, age group) the daily admission chance
+    of an agent that is alive and not in a stay is the table's entry
+    times the yearly multiplier 0.82 (simulation day <= 99; no daily target
+    file, so probabilityMul = 0, simulation.h:196-204, 390-399); the index is
+    type * 28 + (female) * 7 + age group, type 1 for precondition 2
+    (cardiovascular), 2 for 4 (pulmonary), 0 otherwise (here precondition 1
+    reads the general row).  Stay lengths are max(1, geometric(1 / L)) days
+    with L = 5, 2, 5 (the truncated average lengths 5.55, 2.78, 5.24): mean
+    (1 - p)/p + p, one day with chance p + (1 - p) p.  A stay that ends ends
+    in death with the entry 14 further on.  Checked on 10,000 identical
+    agents per cell over 20 days."""
+    from tests.mini_world import HOME, HOSPITAL, MiniWorld, agent_type, event, person, place, schedule
+    n = 10_000
+    t = agent_type(1, [schedule("W", "WEEKENDS", [event(8, 1, 10, 10.1, 1)])])
+    places = [place("H0", HOME), place("HOSP", HOSPITAL), place("S8", 8)]
+    locs = [(HOME, "H0"), (HOSPITAL, "HOSP"), (8, "S8")]
+    keys = list(CELLS)
+    people = [person(1, locs, age=a, sex=s, pre_cond=pc) for (pc, s, a) in keys for _ in range(n)]
+    w = MiniWorld(places, people, [t])
+    args = w.args(weeks=3)
+    args[args.index("--otherDisease") + 1] = "1"
+    o = OracleSim(args)
+    cell = np.repeat(np.arange(len(keys)), n)
+    adm = np.zeros(len(keys))
+    exposure = np.zeros(len(keys))
+    lengths = {0: [], 1: [], 2: []}
+    ends = np.zeros(len(keys))
+    deaths = np.zeros(len(keys))
+    try:
+        dead_state = o.state_names().index("D2")
+        until = o.field("hospitalizedUntilTimestamp").astype(np.int64)
+        alive = np.ones(len(cell), dtype=bool)
+        for day in range(20):
+            ts = day * SPD
+            o.steps(ts + 1 - o.timestamp)
+            ended = alive & (until == ts) & (ts > 0)
+            np.add.at(ends, cell[ended], 1)
+            state = o.field("state")
+            died = ended & (state == dead_state)
+            np.add.at(deaths, cell[died], 1)
+            alive &= state != dead_state
+            eligible = alive & ~(until > ts)
+            np.add.at(exposure, cell[eligible], 1)
+            hts = o.field("hospitalizedTimestamp")
+            new_until = o.field("hospitalizedUntilTimestamp").astype(np.int64)
+            new = (hts == ts) & (new_until > ts) & eligible
+            np.add.at(adm, cell[new], 1)
+            for k in np.nonzero(new)[0][:4000]:
+                pc = keys[cell[k]][0]
+                lengths[1 if pc == "2" else 2 if pc == "4" else 0].append((new_until[k] - ts) // SPD)
+            until = new_until
+    finally:
+        o.close()
+    for c, key in enumerate(keys):
+        exp = exposure[c] * CELLS[key][0] * 0.82
+        assert abs(adm[c] - exp) <= 4.5 * np.sqrt(exp) + 2, (key, adm[c], exp)
+    for typ, p in ((0, 0.2), (1, 0.5), (2, 0.2)):
+        L = np.array(lengths[typ], dtype=np.float64)
+        assert len(L) > 100 and L.min() >= 1
+        mean = (1 - p) / p + p
+        sd = np.sqrt((1 - p) / p ** 2)
+        assert abs(L.mean() - mean) <= 4.5 * sd / np.sqrt(len(L)), (typ, L.mean(), mean)
+        one = p + (1 - p) * p
+        assert abs((L == 1).mean() - one) <= 4.5 * np.sqrt(one * (1 - one) / len(L)), typ
+    exp_deaths = float(sum(ends[c] * CELLS[k][1] for c, k in enumerate(keys)))
+    assert ends.sum() > 1000
+    assert abs(deaths.sum() - exp_deaths) <= 4.5 * np.sqrt(exp_deaths) + 2, (deaths.sum(), exp_deaths)
