@@ -276,6 +276,10 @@ public:
     std::vector<unsigned> variantCounts;
     unsigned healthcareWorkerCount = 0;
     unsigned immunizedToday = 0;
+    // with snapOn, a copy of the whole state at the moment the day's
+    // statistics are computed (for recounting them in the tests)
+    bool snapOn = false;
+    std::shared_ptr<Sim> statSnap;
     // calibration interface: location-type multipliers, expected-infection
     // hazard and its gradient (FP64), hazard sums at perturbed parameters
     std::vector<float> typeMul = std::vector<float>(256, 1.0f);
@@ -1977,6 +1981,7 @@ void Sim::midnight() {
     immunizationUpdate();
     if (sc.opt.otherDisease) otherDisease();
     auto st = refreshStatistics();
+    if (snapOn) statSnap = std::make_shared<Sim>(*this);
     lastStats = st;
     dayStats.push_back(st);
     closureMidnight(st);
@@ -2240,6 +2245,8 @@ const char* oracle_last_error() { return g_err.c_str(); }
 
 void oracle_plant(int code) { oracle::plant = code; }
 
+void oracle_snapshot_stats(void* h, int on) { ((oracle::Sim*)h)->snapOn = on != 0; }
+
 void oracle_destroy(void* h) { delete (oracle::Sim*)h; }
 
 // Advances n iterations of runSimulation's loop; returns 0 or -1 on error.
@@ -2305,6 +2312,11 @@ int oracle_day_stats(void* h, int d, unsigned* out, int cap) {
 int oracle_field(void* h, const char* name, unsigned* out) {
     auto* s = (oracle::Sim*)h;
     std::string n = name;
+    if (n.rfind("snap:", 0) == 0) {  // the copy taken when the last statistics were computed
+        if (!s->statSnap) return -1;
+        s = s->statSnap.get();
+        n = n.substr(5);
+    }
     unsigned N = s->N, L = s->L;
     auto fbits = [](float f) {
         unsigned u;
@@ -2325,7 +2337,12 @@ int oracle_field(void* h, const char* name, unsigned* out) {
     else if (n == "stayedHome") for (unsigned i = 0; i < N; ++i) out[i] = s->stayedHome[i];
     else if (n == "noWorkAgent") for (unsigned i = 0; i < N; ++i) out[i] = s->noWorkAgent[i];
     else if (n == "lastTest") for (unsigned i = 0; i < N; ++i) out[i] = s->lastTest[i];
-    else if (n == "susceptible0") for (unsigned i = 0; i < N; ++i) out[i] = fbits(s->pp[i].susceptible[0]);
+    else if (n.rfind("susceptible", 0) == 0 && n.size() == 12 && n[11] >= '0' && n[11] < '0' + oracle::MAXS) {
+        unsigned v = (unsigned)(n[11] - '0');
+        for (unsigned i = 0; i < N; ++i) out[i] = fbits(s->pp[i].susceptible[v]);
+    }
+    else if (n == "healthcareWorker") for (unsigned i = 0; i < N; ++i) out[i] = s->hcw[i];
+    else if (n == "timestamp") out[0] = s->timestamp;
     else if (n == "infectedTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].infectedTimestamp;
     else if (n == "infectedLocation") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].infectedLocation;
     else if (n == "worstStateTimestamp") for (unsigned i = 0; i < N; ++i) out[i] = s->stats[i].worstStateTimestamp;

@@ -75,6 +75,7 @@ def lib():
     L.oracle_geometric.argtypes = [ctypes.c_ulonglong, ctypes.c_uint32, ctypes.c_double]
     L.oracle_geometric.restype = ctypes.c_int
     L.oracle_plant.argtypes = [ctypes.c_int]
+    L.oracle_snapshot_stats.argtypes = [ctypes.c_void_p, ctypes.c_int]
     for f in ("oracle_num_events", "oracle_num_offsets"):
         getattr(L, f).argtypes = [ctypes.c_void_p]
         getattr(L, f).restype = ctypes.c_uint32
@@ -148,11 +149,17 @@ class OracleSim:
         return list(out[:n])
 
     def field(self, name: str) -> np.ndarray:
-        n = self.n_locations if name in LOCATION_FIELDS else self.n_agents
+        base = name[5:] if name.startswith("snap:") else name
+        n = self.n_locations if base in LOCATION_FIELDS else max(1, self.n_agents)
         out = np.zeros(n, dtype=np.uint32)
         if lib().oracle_field(self._h, name.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))) != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         return out
+
+    def snapshot_stats(self, on: bool = True):
+        """Keeps a copy of the state taken when each day's statistics are
+        computed; field("snap:<name>") reads the latest copy."""
+        lib().oracle_snapshot_stats(self._h, 1 if on else 0)
 
     def location_sums(self, variant: int) -> np.ndarray:
         out = np.zeros(self.n_locations, dtype=np.int64)
