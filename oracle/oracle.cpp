@@ -1162,6 +1162,7 @@ void Sim::planLocations() {
 
 // testingPolicies.h:59-281, 370-475
 void Sim::performTests() {
+    if (sc.opt.testingPolicy == "none") return;  // NoTesting: no tests, statistics stay 0 (testingPolicies.h:11-20)
     std::fill(locFlags.begin(), locFlags.end(), (uint8_t)0);
     const unsigned day = 24 * 60 / dt;
     const unsigned qlen = quarantineLength * 24 * 60 / dt;
@@ -1723,6 +1724,7 @@ void Sim::countVariantCases() {
 
 // closurePolicies.h:84-479
 void Sim::initClosures() {
+    if (sc.opt.closurePolicy == "none") return;  // NoClosure::init reads no rules (closurePolicies.h:41)
     unsigned enable = sc.opt.enableClosures;
     std::vector<unsigned> hospitalStates;
     for (unsigned i = 0; i < header.size(); ++i)
@@ -1971,6 +1973,7 @@ void Sim::initClosures() {
 }
 
 void Sim::closureMidnight(std::vector<unsigned>& st) {
+    if (sc.opt.closurePolicy == "none") return;  // NoClosure::midnight (closurePolicies.h:42)
     for (auto& c : conds) c.active = c.condition(&c, st);
     for (auto& r : rules) r.rule(&r);
 }

@@ -92,6 +92,8 @@ std::vector<OptSpec> option_table(Options& o) {
         {"", "synthSeed", OptSpec::U64, &o.synthSeed},
         {"", "synthInfected", OptSpec::D, &o.synthInfected},
         {"", "movementPolicy", OptSpec::S, &o.movementPolicy},
+        {"", "testingPolicy", OptSpec::S, &o.testingPolicy},
+        {"", "closurePolicy", OptSpec::S, &o.closurePolicy},
         {"h", "help", OptSpec::B, &o.help},
         {"", "version", OptSpec::B, &o.version},
     };
@@ -178,6 +180,10 @@ Options parse_options(int argc, const char* const* argv, bool skip_program_name)
     }
     if (o.movementPolicy != "real" && o.movementPolicy != "dummy" && o.movementPolicy != "none")
         throw InitError(ERR_BAD_ARG, "bad value for --movementPolicy: " + o.movementPolicy + " (real, dummy or none)");
+    if (o.testingPolicy != "detailed" && o.testingPolicy != "none")
+        throw InitError(ERR_BAD_ARG, "bad value for --testingPolicy: " + o.testingPolicy + " (detailed or none)");
+    if (o.closurePolicy != "rule" && o.closurePolicy != "none")
+        throw InitError(ERR_BAD_ARG, "bad value for --closurePolicy: " + o.closurePolicy + " (rule or none)");
     return o;
 }
 

@@ -119,6 +119,12 @@ struct Options {
     // NoMovement (movementPolicies.h:9-66, 1339-1752), which the reference
     // selects at build time (CMakeLists.txt:76-85)
     std::string movementPolicy = "real";
+    // --testingPolicy detailed|none: DetailedTesting or NoTesting
+    // (testingPolicies.h:11-20, 292-476), and --closurePolicy rule|none:
+    // RuleClosure or NoClosure (closurePolicies.h:36-46, 47-481), chosen at
+    // build time in the reference (CMakeLists.txt:76-85, configTypes.h.in:15)
+    std::string testingPolicy = "detailed";
+    std::string closurePolicy = "rule";
     bool help = false;
     bool version = false;
 };

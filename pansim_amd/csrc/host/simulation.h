@@ -995,9 +995,66 @@ public:
     }
 };
 
+// DetailedTesting or NoTesting, chosen at run time (--testingPolicy).
+template <typename SimulationType>
+class SelectableTesting : public DetailedTesting<SimulationType> {
+    using Detailed = DetailedTesting<SimulationType>;
+    NoTesting<SimulationType> none_;
+    bool detailed_ = true;
+
+public:
+    void initializeArgs(const Options& o) {
+        detailed_ = o.testingPolicy != "none";
+        if (detailed_) Detailed::initializeArgs(o);
+        else none_.initializeArgs(o);
+    }
+    void init(const LocationTypeIds& lt) {
+        if (detailed_) Detailed::init(lt);
+        else none_.init(lt);
+    }
+    void performTests(Timehandler simTime, unsigned timeStep) {
+        if (detailed_) Detailed::performTests(simTime, timeStep);
+        else none_.performTests(simTime, timeStep);
+    }
+    std::tuple<unsigned, unsigned, unsigned> getStats() { return detailed_ ? Detailed::getStats() : none_.getStats(); }
+    // NoTesting has no probabilities to update: the TPxxx flags change nothing
+    void updateTestingProbs(const std::string& s) {
+        if (detailed_) Detailed::updateTestingProbs(s);
+    }
+};
+
+// RuleClosure or NoClosure, chosen at run time (--closurePolicy).  NoClosure
+// reads no rules at all, so none of them acts, not even the ones that act
+// with --enableClosures 0.
+template <typename SimulationType>
+class SelectableClosure : public RuleClosure<SimulationType> {
+    using Rules = RuleClosure<SimulationType>;
+    NoClosure<SimulationType> none_;
+    bool rules_ = true;
+
+public:
+    void initializeArgs(const Options& o) {
+        rules_ = o.closurePolicy != "none";
+        if (rules_) Rules::initializeArgs(o);
+        else none_.initializeArgs(o);
+    }
+    void init(const LocationTypeIds& lt, const std::vector<ClosureRuleRaw>& input, const std::string& header) {
+        if (rules_) Rules::init(lt, input, header);
+        else none_.init(lt, input, header);
+    }
+    void midnight(Timehandler t, unsigned dt, std::vector<unsigned>& stats) {
+        if (rules_) Rules::midnight(t, dt, stats);
+        else none_.midnight(t, dt, stats);
+    }
+    void step(Timehandler t, unsigned dt) {
+        if (rules_) Rules::step(t, dt);
+        else none_.step(t, dt);
+    }
+};
+
 // The reference's concrete binding (cmake/config/in/configTypes.h.in:10-16),
-// with the movement policy selectable at run time.
+// with the movement, testing and closure policies selectable at run time.
 using ConfiguredSimulation = Simulation<std::array<double, 2>, unsigned, DynamicPPState, BasicAgentMeta,
-                                        SelectableMovement, BasicInfection, DetailedTesting, RuleClosure>;
+                                        SelectableMovement, BasicInfection, SelectableTesting, SelectableClosure>;
 
 }  // namespace ps

@@ -95,3 +95,20 @@ def test_parity_tiny_ragged_and_extreme_populations(n, infected):
     # pass at all) and everybody infected at the start
     args = pansim_amd.synthetic_args(n, 1, 3, 5, extra=["-k", "0.3", "--synthInfected", str(infected)])
     run_parity(args, [1, 70, 144, 145, 300])
+
+
+@pytest.mark.parametrize("flags", [["--testingPolicy", "none"], ["--closurePolicy", "none"],
+                                   ["--testingPolicy", "none", "--closurePolicy", "none"]])
+def test_parity_no_testing_and_no_closure(flags):
+    # NoTesting / NoClosure (testingPolicies.h:11-20, closurePolicies.h:36-46)
+    # through --testingPolicy / --closurePolicy, with the shipped closure rules
+    args = pansim_amd.synthetic_args(3000, 1, 9, 5, extra=["-k", "0.02", "--synthInfected", "0.02", *flags])
+    run_parity(args, [1, 144, 145, 600, 1008])
+
+
+def test_parity_config1_full_size():
+    # the CPU-runnable workload at its size: -n 100000 -N 20000 (96,786 agents
+    # after the tourists are skipped, 20,001 locations), two weeks, seed 0;
+    # every field and every day's statistics at mid-day, midnight and
+    # activity-change checkpoints and at the end
+    run_parity(pansim_amd.config1_args(), [1, 48, 145, 500, 1008, 1009, 1500, 2016])

@@ -56,3 +56,36 @@ def test_no_movement_keeps_every_location():
     start = o.field("location").copy()
     o.steps(2 * 144)
     assert np.array_equal(o.field("location"), start)
+
+
+def test_no_testing_and_no_closure_policies():
+    """NoTesting (testingPolicies.h:11-20): no agent is ever tested and the
+    three testing entries of the statistics stay 0.  NoClosure
+    (closurePolicies.h:36-46): no rule is read, so with the shipped closure
+    rules (which close schools and workplaces from day 0 and put masks on)
+    no location is ever closed and every infectiousness stays as loaded;
+    with RuleClosure the same run does close locations."""
+    base = pansim_amd.synthetic_args(3000, 1, 2, 2, extra=["-k", "0.01", "--synthInfected", "0.02"])
+    o = OracleSim(pansim_amd.with_data_dir(base + ["--testingPolicy", "none"]))
+    try:
+        o.steps(7 * 144)
+        assert (o.field("lastTest") == 0xFFFFFFFF).all() or (o.field("lastTest") == 0).all()
+        for d in range(o.num_days):
+            assert o.day_stats(d)[13:16] == [0, 0, 0]
+    finally:
+        o.close()
+    ref = OracleSim(pansim_amd.with_data_dir(base))
+    none = OracleSim(pansim_amd.with_data_dir(base + ["--closurePolicy", "none"]))
+    try:
+        inf0 = none.field("locInfectiousness").copy()
+        closed_rule = 0
+        for day in range(1, 7):
+            ref.steps(day * 144 + 1 - ref.timestamp)
+            none.steps(day * 144 + 1 - none.timestamp)
+            assert (none.field("locOpen") == 1).all()
+            assert np.array_equal(none.field("locInfectiousness"), inf0)
+            closed_rule += int((ref.field("locOpen") == 0).sum())
+        assert closed_rule > 0
+    finally:
+        ref.close()
+        none.close()
