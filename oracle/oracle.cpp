@@ -64,6 +64,11 @@
 
 #include "../pansim_amd/csrc/host/scenario.h"
 #include "ref_rng.h"
+#ifdef ORACLE_OMP
+#include <omp.h>
+
+#include <parallel/algorithm>
+#endif
 
 namespace oracle {
 
@@ -78,6 +83,11 @@ namespace oracle {
 //  7 the last eligible adult of a home takes the child care (:1149-1155)
 //  8 one quarantined classroom already quarantines the school (:1297)
 int plant = 0;
+
+// quarantineLocations pending for the agent being moved (per thread: the
+// OpenMP build moves agents in parallel)
+thread_local bool qLoc = false;
+thread_local unsigned qLocUntil = 0, qLocPrev = 0;
 
 using ps::Scenario;
 constexpr unsigned UMAX = 0xFFFFFFFFu;
@@ -278,6 +288,7 @@ public:
     unsigned immunizedToday = 0;
     // with snapOn, a copy of the whole state at the moment the day's
     // statistics are computed (for recounting them in the tests)
+    std::vector<unsigned> sortedAgents, sortKey, locOffsets;  // OpenMP build: the per-step location index
     bool snapOn = false;
     std::shared_ptr<Sim> statSnap;
     // calibration interface: location-type multipliers, expected-infection
@@ -327,8 +338,7 @@ public:
     void quarantineLocations(unsigned i, unsigned until, unsigned previousQuarantineUntil,
                              std::vector<std::pair<unsigned, unsigned>>& writes, Rng& rng);
     void doMovementRoutine(unsigned i, unsigned day, TD simTime, Rng& rng);
-    bool qLoc = false;  // quarantineLocations pending for the agent being moved
-    unsigned qLocUntil = 0, qLocPrev = 0;
+
     void doMovement(unsigned i, unsigned day, TD simTime, std::vector<std::pair<unsigned, unsigned>>& writes);
     void movement();
     void rebuildLocationSums();
@@ -953,6 +963,28 @@ void Sim::movement() {
     }
     TD simTime = TD::fromMinutes((timestamp % stepsPerDay) * dt);
     unsigned day = holidayModeActive ? 6u : dayOfWeek();
+#ifdef ORACLE_OMP
+    // the reference's OpenMP path: one parallel for over the agents
+    // (movementPolicies.h:1717-1719); each thread keeps its location writes
+    // in agent order and the static schedule hands out ascending agent
+    // ranges, so concatenating them in thread order keeps the agent order
+    std::vector<std::vector<std::pair<unsigned, unsigned>>> perThread(omp_get_max_threads());
+#pragma omp parallel
+    {
+        auto& mine = perThread[omp_get_thread_num()];
+        std::vector<std::pair<unsigned, unsigned>> w;
+#pragma omp for schedule(static)
+        for (unsigned i = 0; i < N; ++i) {
+            w.clear();
+            unsigned old = location[i];
+            doMovement(i, day, simTime, w);
+            if (location[i] == UMAX) location[i] = old;
+            mine.insert(mine.end(), w.begin(), w.end());
+        }
+    }
+    for (auto& v : perThread)
+        for (auto& x : v) quarantineUntil[x.first] = x.second;
+#else
     std::vector<std::pair<unsigned, unsigned>> all, w;
     for (unsigned i = 0; i < N; ++i) {
         w.clear();
@@ -962,11 +994,45 @@ void Sim::movement() {
         all.insert(all.end(), w.begin(), w.end());
     }
     for (auto& x : all) quarantineUntil[x.first] = x.second;  // ascending agent order: last writer wins
+#endif
 }
 
 // Per-location occupancy and infectiousness sums (updatePerLocationAgentLists
 // + reduce_by_location), rebuilt from scratch.
 void Sim::rebuildLocationSums() {
+#ifdef ORACLE_OMP
+    // the reference's OpenMP structure (util.cpp:55-110, util.h:47-103):
+    // agents stably sorted by location every step, offsets from the sorted
+    // keys, then each location's sum taken sequentially over its agents in
+    // ascending agent order (the fixed-point sum of the sequential build).
+    // With one thread the reference sorts nothing (simulation.h:720-723) and
+    // scatters in agent order, as below.
+    if (omp_get_max_threads() > 1) {
+    sortedAgents.resize(N);
+    sortKey.resize(N);
+#pragma omp parallel for schedule(static)
+    for (unsigned i = 0; i < N; ++i) sortedAgents[i] = i;
+    __gnu_parallel::stable_sort(sortedAgents.begin(), sortedAgents.end(),
+                                [this](unsigned a, unsigned b) { return location[a] < location[b]; });
+#pragma omp parallel for schedule(static)
+    for (unsigned i = 0; i < N; ++i) sortKey[i] = location[sortedAgents[i]];
+    locOffsets.assign(L + 1, 0u);
+#pragma omp parallel for schedule(static)
+    for (unsigned l = 0; l <= L; ++l)
+        locOffsets[l] = (unsigned)(std::lower_bound(sortKey.begin(), sortKey.end(), l) - sortKey.begin());
+#pragma omp parallel for schedule(dynamic, 256)
+    for (unsigned l = 0; l < L; ++l) {
+        locCount[l] = locOffsets[l + 1] - locOffsets[l];
+        for (unsigned v = 0; v < nVar; ++v) locSum[(size_t)v * L + l] = 0;
+        for (unsigned k = locOffsets[l]; k < locOffsets[l + 1]; ++k) {
+            const unsigned i = sortedAgents[k];
+            float x = pp[i].infectious;
+            if (x != 0.0f) locSum[(size_t)pp[i].variant * L + l] += (int64_t)((double)x * 4294967296.0);
+        }
+    }
+    return;
+    }
+#endif
     std::fill(locCount.begin(), locCount.end(), 0u);
     std::fill(locSum.begin(), locSum.end(), (int64_t)0);
     for (unsigned i = 0; i < N; ++i) {
@@ -1007,6 +1073,9 @@ void Sim::infectionsAtLocations(unsigned variant) {
     float tmpK = (float)k;
     float season = (float)seasonality();
     const int64_t* sums = &locSum[(size_t)variant * L];
+#ifdef ORACLE_OMP
+#pragma omp parallel for schedule(static)
+#endif
     for (unsigned l = 0; l < L; ++l) {
         float num = (float)((double)sums[l] * (1.0 / 4294967296.0));
         if (num == 0.0f) {
@@ -1041,6 +1110,10 @@ void Sim::infectionsAtLocations(unsigned variant) {
         }
     }
     std::vector<unsigned> newlyInfected;  // ascending agent index
+#ifdef ORACLE_OMP
+    const bool dump = !sc.opt.dumpLocationInfectiousList.empty();
+#pragma omp parallel for schedule(static) if (!dump)
+#endif
     for (unsigned i = 0; i < N; ++i) {
         PPState& p = pp[i];
         if (p.susceptible[variant] > 0.0f) {
@@ -1048,7 +1121,10 @@ void Sim::infectionsAtLocations(unsigned variant) {
             if (rl == 0.0f) continue;  // u < 0 never holds; the draw is counter-addressed
             Rng rng(seed, i, timestamp, PHASE_INFECT + variant);
             if (rng.unit() < (double)(rl * p.susceptible[variant])) {
-                newlyInfected.push_back(i);
+#ifdef ORACLE_OMP
+                if (dump)
+#endif
+                    newlyInfected.push_back(i);
                 gotInfected(p, (uint8_t)variant);
                 Stats& st = stats[i];
                 st.infectedTimestamp = timestamp;
@@ -1239,6 +1315,9 @@ void Sim::performTests() {
 
 // simulation.h:425-471 + dynamicPPState.cpp:246-303 + multiBadMatrix.cpp:120-125
 void Sim::updateAgents() {
+#ifdef ORACLE_OMP
+#pragma omp parallel for schedule(static)
+#endif
     for (unsigned i = 0; i < N; ++i) {
         PPState& p = pp[i];
         Meta& m = meta[i];

@@ -87,13 +87,18 @@ def build_pypansim(verbose=False):
 
 
 def build_oracle(verbose=False):
+    """The oracle twice: liboracle.so (sequential, the checker of the tests)
+    and liboracle_omp.so (the same code with the reference's OpenMP structure:
+    parallel movement, infection and progression, a per-step stable sort by
+    location and per-location sums; the CPU baseline of bench.py)."""
     os.makedirs(ORACLE_OUT, exist_ok=True)
     srcs = [os.path.join(ORACLE_DIR, "oracle.cpp")] + [os.path.join(CSRC, "host", s) for s in INPUT_SOURCES]
-    lib = os.path.join(ORACLE_OUT, "liboracle.so")
-    tmp = lib + ".tmp"
-    _run([CXX, *CXXFLAGS, "-shared", "-o", tmp, *srcs], verbose)
-    os.replace(tmp, lib)
-    return lib
+    for name, extra in (("liboracle.so", []), ("liboracle_omp.so", ["-O3", "-fopenmp", "-DORACLE_OMP"])):
+        lib = os.path.join(ORACLE_OUT, name)
+        tmp = lib + ".tmp"
+        _run([CXX, *CXXFLAGS, *extra, "-shared", "-o", tmp, *srcs], verbose)
+        os.replace(tmp, lib)
+    return os.path.join(ORACLE_OUT, "liboracle.so")
 
 
 def build_all(verbose=False):
