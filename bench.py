@@ -3,38 +3,58 @@
 
 Workload: 10M-agent synthetic household population (schools, classrooms,
 lognormal workplaces, shops, leisure, doctors, hospitals), 4 weeks at
-dt = 10 min (4032 steps), k = 3.74e-4, 0.1% initially infected, all policies
-on (DetailedTesting, quarantinePolicy 3, RuleClosure with the default
-closureRules.json, non-COVID hospitalisation, immunity waning).
+dt = 10 min (4032 steps),
+k = 3.74e-4, 0.1% initially infected, all policies on (DetailedTesting,
+quarantinePolicy 3, RuleClosure with the default closureRules.json, non-COVID
+hospitalisation, immunity waning).
 
 A "step" is one iteration of the reference's runSimulation loop: the
 midnight block when due (testing, progression, immunity, other disease,
 statistics, closures, location planning), then movement and infection for
-every agent.  --warmup steps run untimed first (default: day 0), then exactly
---steps steps are timed between two device synchronisations (default: the
-remaining 27 days).  value = agents * steps / seconds (agent-steps/s).
+every agent.  value = agents * steps / seconds (agent-steps/s) over exactly
+--steps timed steps between two device synchronisations.
 
-roofline: the per-step pipeline (movement kernel, then the infection kernel,
-which also applies the movement's deferred location quarantine writes) is
-the dominant work; its algorithmic bytes per step are
-22 B/agent + 24 B/location + 4 B/relocated agent, divided by its device time
-from HIP events on the context stream during the timed region: one pair per
-day around the run of movement and infection launches between two midnight
-blocks (inter-kernel gaps included, midnight work excluded).  The warmup
-runs with an event pair per kernel launch instead, for the movement /
-infection split.  traffic: HBM bytes per step of the
-same kernels from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-(profiles/traffic_synth10M.json, 10M-agent workload only), else null.
+Which steps are timed:
+  * --steps >= 144 (default 3888 = days 1-27 after --warmup 144): the run
+    from step --warmup on;
+  * --steps < 144: a window inside the 4-week run, starting at
+    --window-day / --window-time (default day 14, 07:00, the morning of the
+    third week: the epidemic is under way and agents change activities).  The
+    steps before it run untimed (spin-up, reported with its wall time), then
+    --warmup untimed steps, then the window.  The window is named in
+    config.workload.
+A sub-record "full_day" times one complete simulated day (its midnight block
+and 144 steps) after the timed steps, for the day-average picture.
 
-cpu_baseline: the oracle (sequential C++ re-statement of the reference
-per-step pipeline) on a bounded sample of the same workload (1M synthetic
-agents, one simulated week: 1008 steps), on rank 0 only.
+roofline (the dominant kernel, k_move_tile, the movement step): per launch
+the algorithmic bytes of the movement share of the per-step byte model,
+8 B per agent (the reference's stepsUntilMove countdown, read and written)
++ 4 B per relocated agent (its new location), divided by the kernel's
+average launch duration from a HIP event pair around every movement launch
+of the timed steps (on the context stream).  traffic: HBM bytes per launch
+of k_move_tile from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over the same
+window, copy-calibrated, read from profiles/traffic_<workload>.json when
+that file names this exact workload and window, else null.
+"step_pipeline" reports the movement + infection pipeline against the whole
+per-step byte model 22 B/agent + 24 B/location + 4 B/relocated agent
+(full_day record), and "movement_design" the bytes this design's movement
+kernel itself must move (2 B/agent wake scan, 62 B per evaluated agent,
+48 B per relocation).
+
+cpu_baseline: the OpenMP build of the oracle (the reference's CPU structure:
+parallel movement, a per-step stable sort by location, per-location sums in
+ascending agent order) on the CPU-runnable workload at its size
+(-n 100000 -N 20000, 2 weeks), on all host threads with OMP_PROC_BIND=true,
+and with 1 thread (no sort, as the reference with one thread).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -42,6 +62,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STEPS_PER_DAY = 144
 
 
 def parse():
@@ -52,9 +73,11 @@ def parse():
     ap.add_argument("--agents", type=int, default=10_000_000)
     ap.add_argument("--weeks", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--window-day", type=int, default=14)
+    ap.add_argument("--window-time", default="07:00")
+    ap.add_argument("--no-full-day", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-agents", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=1008)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all host threads)")
     ap.add_argument("--ensemble-replicas", type=int, default=64,
                     help="replicas of the batched-ensemble measurement (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=288)
@@ -65,6 +88,36 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Roctx:
+    """roctxProfilerResume / Pause around the timed steps, so that
+    `rocprofv3 --selected-regions` records exactly them; no-ops when the
+    library is absent or no profiler is attached."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so", "/opt/rocm/lib/librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for f in ("roctxProfilerResume", "roctxProfilerPause"):
+                getattr(self.lib, f).argtypes = [ctypes.c_uint64]
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
 
 
 def copy_probe(mib):
@@ -81,26 +134,59 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "repeats": 5}), flush=True)
 
 
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+CPU_SCRIPT = r"""
+import sys, time, json
+sys.path.insert(0, {root!r})
+import pansim_amd
+from oracle.oracle import OracleSim
+o = OracleSim(pansim_amd.with_data_dir(pansim_amd.config1_args()), openmp=True)
+t0 = time.perf_counter()
+o.steps(2016)
+dt = time.perf_counter() - t0
+print(json.dumps({{"agents": o.n_agents, "steps": 2016, "seconds": dt}}))
+"""
+
+
+def cpu_run(threads):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="true")
+    r = subprocess.run([sys.executable, "-c", CPU_SCRIPT.format(root=ROOT)], env=env, capture_output=True, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline(a):
-    from oracle.oracle import OracleSim
-    import pansim_amd
-
-    args = pansim_amd.synthetic_args(a.cpu_agents, 1, a.seed, 1)
-    o = OracleSim(args)
-    t0 = time.perf_counter()
-    o.steps(a.cpu_steps)
-    dt = time.perf_counter() - t0
+    # the host share of the GPU's box (OMP_NUM_THREADS there), else every thread
+    threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    log(f"cpu baseline: config 1 on {threads} threads ...")
+    many = cpu_run(threads)
+    log(f"cpu baseline: {many['seconds']:.1f}s; 1 thread ...")
+    one = cpu_run(1)
+    rate = lambda r: r["agents"] * r["steps"] / r["seconds"]
     return {
-        "value": o.n_agents * a.cpu_steps / dt,
+        "value": rate(many),
         "unit": "agent-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"oracle (sequential C++ re-statement of the reference pipeline), {o.n_agents} synthetic agents, "
-                  f"{a.cpu_steps} steps incl. the day-0 midnight block, {dt:.1f} s",
+        "sample": f"OpenMP oracle restatement of the reference's CPU path (parallel movement, per-step stable sort by "
+                  f"location, per-location sums) on the CPU-runnable workload at its size: -n 100000 -N 20000 "
+                  f"({many['agents']} agents after tourists are skipped, 20,001 locations), 2 weeks = 2016 steps, "
+                  f"{threads} threads with OMP_PROC_BIND=true: {many['seconds']:.1f} s",
+        "one_thread_value": rate(one),
+        "one_thread_seconds": one["seconds"],
+        "cpu_model": cpu_model(),
     }
 
 
@@ -205,6 +291,42 @@ def gradient_measure(a):
     }
 
 
+def clock(ts):
+    m = (ts % STEPS_PER_DAY) * 10
+    return f"day {ts // STEPS_PER_DAY} {m // 60:02d}:{m % 60:02d}"
+
+
+def steps_timed(sim, n, roctx=None):
+    """n steps between two synchronisations; returns the wall seconds."""
+    sim.sync()
+    if roctx:
+        roctx.resume()
+    t0 = time.perf_counter()
+    done = 0
+    while done < n:
+        k = min(STEPS_PER_DAY, n - done)
+        sim.steps(k)
+        done += k
+        if done % (7 * STEPS_PER_DAY) == 0:
+            log(f"  {done}/{n} steps")
+    sim.sync()
+    dt = time.perf_counter() - t0
+    if roctx:
+        roctx.pause()
+    return dt
+
+
+def traffic_for(workload_key):
+    path = os.path.join(ROOT, "profiles", "traffic_k_move_tile_synth10M.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    if t.get("workload_key") != workload_key:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -216,101 +338,142 @@ def main():
         sys.exit(2)
     import pansim_amd
 
-    total = a.weeks * 7 * 144
-    if a.warmup + a.steps > total:
-        a.steps = max(1, total - a.warmup)
+    roctx = Roctx()
+    roctx.pause()  # only the timed steps are profiled under --selected-regions
+    total = a.weeks * 7 * STEPS_PER_DAY
+    K, W = a.steps, a.warmup
     args = pansim_amd.synthetic_args(a.agents, a.weeks, a.seed, 1)
     t0 = time.perf_counter()
     sim = pansim_amd.Simulation(args)
     log(f"built {sim.n_agents} agents, {sim.n_locations} locations in {time.perf_counter() - t0:.1f}s")
-    # the warmup runs with one event pair per kernel launch, for the split of
-    # the step between movement and infection (informational)
-    sim.enable_timing(True)
-    sim.steps(a.warmup)
-    sim.sync()
-    wm, wmove_ms, wi, winfect_ms = sim.timing_summary()
-    log(f"warmup {a.warmup} steps done (t={sim.timestamp})")
-    sim.movement_counters(reset=True)
-    # timed region: one event pair per day around its movement + infection launches
-    sim.enable_timing(True, pipeline=True)
-    sim.sync()
-    t1 = time.perf_counter()
-    done = 0
-    chunk = 144
-    while done < a.steps:
-        n = min(chunk, a.steps - done)
-        sim.steps(n)
-        done += n
-        if done % (7 * 144) == 0:
-            log(f"  {done}/{a.steps} steps")
-    sim.sync()
-    elapsed = time.perf_counter() - t1
-    n_pipe, pipe_ms, _, _ = sim.timing_summary()
-    evaluated, relocated = sim.movement_counters()
     N, L = sim.n_agents, sim.n_locations
-    K = a.steps
-    value = N * K / elapsed
-    # algorithmic bytes of the per-step pipeline (movement + infection)
-    step_bytes_total = (22.0 * N + 24.0 * L) * K + 4.0 * relocated
-    pipeline_s = pipe_ms / 1e3
-    achieved = step_bytes_total / pipeline_s / 1e9 if pipeline_s > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_synth10M.json")
-    if N == 10_000_000 and os.path.exists(tpath):
-        # HBM bytes per step of the same kernels, from the rocprofv3 FETCH_SIZE /
-        # WRITE_SIZE passes recorded in that file (copy-calibrated)
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_step")
-    last = sim.day_stats(sim.num_days - 1)
+    window = K < STEPS_PER_DAY
+    if window:
+        hh, mm = (int(x) for x in a.window_time.split(":"))
+        start = a.window_day * STEPS_PER_DAY + (hh * 60 + mm) // 10
+        start = max(W, min(start, total - K - 2 * STEPS_PER_DAY))
+        spin = start - W
+    else:
+        spin = 0
+        K = min(K, total - W)
+        start = W
+    t1 = time.perf_counter()
+    sim.steps(spin)
+    sim.sync()
+    spin_s = time.perf_counter() - t1
+    if spin:
+        log(f"spin-up: {spin} untimed steps to {clock(spin)} in {spin_s:.2f}s")
+    sim.steps(W)
+    sim.sync()
+    log(f"warmup {W} steps done ({clock(sim.timestamp)})")
+
+    # timed steps: one HIP event pair around every movement launch
+    sim.movement_counters(reset=True)
+    sim.enable_timing(True, movement_only=True)
+    elapsed = steps_timed(sim, K, roctx)
+    n_move, move_ms, _, _ = sim.timing_summary()
+    sim.enable_timing(False)
+    evaluated, relocated = sim.movement_counters()
+    st = sim.day_stats(sim.num_days - 1)
+    end = sim.timestamp
+    window_text = f"steps {start}..{end - 1} ({clock(start)} to {clock(end - 1)})"
+    log(f"timed {K} steps, {window_text}: {elapsed * 1e3 / K:.4f} ms/step")
+
+    move_avg_s = move_ms / 1e3 / max(1, n_move)
+    move_bytes = 8.0 * N + 4.0 * relocated / max(1, K)
+    achieved = move_bytes / move_avg_s / 1e9 if move_avg_s > 0 else 0.0
+    design_bytes = 2.0 * N + 62.0 * evaluated / max(1, K) + 48.0 * relocated / max(1, K)
+    workload_key = f"synth{N}_w{a.weeks}_s{a.seed}_from{start}_k{K}"
+
+    full_day = None
+    if not a.no_full_day and sim.timestamp + 2 * STEPS_PER_DAY <= total:
+        # the next complete simulated day: its midnight block and 144 steps
+        nxt = (sim.timestamp // STEPS_PER_DAY + 1) * STEPS_PER_DAY
+        sim.steps(nxt - sim.timestamp)
+        sim.movement_counters(reset=True)
+        sim.enable_timing(True, pipeline=True)
+        day_s = steps_timed(sim, STEPS_PER_DAY)
+        n_pipe, pipe_ms, _, _ = sim.timing_summary()
+        sim.enable_timing(False)
+        ev_d, rel_d = sim.movement_counters()
+        b_step = 22.0 * N + 24.0 * L + 4.0 * rel_d / STEPS_PER_DAY
+        pipe_s = pipe_ms / 1e3 / max(1, n_pipe)
+        full_day = {
+            "day": nxt // STEPS_PER_DAY,
+            "agent_steps_per_s": N * STEPS_PER_DAY / day_s,
+            "ms_per_step": day_s * 1e3 / STEPS_PER_DAY,
+            "pipeline_ms_per_step": pipe_ms / max(1, n_pipe),
+            "midnight_and_gaps_ms_per_step": (day_s * 1e3 - pipe_ms) / STEPS_PER_DAY,
+            "movers_evaluated_per_step": ev_d / STEPS_PER_DAY,
+            "relocated_per_step": rel_d / STEPS_PER_DAY,
+            "step_pipeline": {
+                "bytes_per_step": b_step,
+                "achieved_GBs": b_step / pipe_s / 1e9 if pipe_s > 0 else 0.0,
+                "frac": b_step / pipe_s / 1e9 / HBM_PEAK_GBS if pipe_s > 0 else 0.0,
+                "model": "22 B/agent + 24 B/location + 4 B/relocated agent per step (movement + infection)",
+            },
+        }
+        log(f"full day {full_day['day']}: {full_day['ms_per_step']:.4f} ms/step, pipeline "
+            f"{full_day['pipeline_ms_per_step']:.4f} ms/step")
     sim.close()
-    ensemble = None
-    if a.ensemble_replicas > 0:
-        ensemble = ensemble_measure(a)
-    gradient = None
-    if a.gradient_replicas > 0:
-        gradient = gradient_measure(a)
-    cpu = None
-    if not a.no_cpu_baseline:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(a)
+
+    ensemble = ensemble_measure(a) if a.ensemble_replicas > 0 else None
+    gradient = gradient_measure(a) if a.gradient_replicas > 0 else None
+    cpu = cpu_baseline(a) if not a.no_cpu_baseline else None
+
+    if window:
+        workload = (f"plain GPU drop-in, {N}-agent synthetic household population, {a.weeks}-week run at dt=10 min, "
+                    f"k=3.74e-4, 0.1% infected at start, all policies on; timed window {window_text} after "
+                    f"{spin} untimed spin-up steps and {W} warmup steps")
+    else:
+        workload = (f"plain GPU drop-in, {N}-agent synthetic household population, {a.weeks} weeks at dt=10 min, "
+                    f"k=3.74e-4, 0.1% infected at start, all policies on; timed {window_text} after {W} warmup steps")
     line = {
         "metric": "agent-steps/s",
-        "value": value,
+        "value": N * K / elapsed,
         "unit": "agent-steps/s",
         "n_gpus": 1,
         "steps": K,
-        "warmup": a.warmup,
+        "warmup": W,
         "ms_per_step": elapsed * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+int64",
+        "dtype": "int32+fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"plain GPU drop-in, {N}-agent synthetic household population, {a.weeks} weeks at dt=10 min, "
-                        "k=3.74e-4, 0.1% infected at start, all policies on",
+            "workload": workload,
             "n_agents": N,
             "n_locations": L,
+            "window_first_step": start,
+            "spinup_steps": spin,
+            "spinup_seconds": spin_s,
             "parallelism": "1 GPU",
         },
         "roofline": {
             "bound": "hbm",
+            "kernel": "k_move_tile",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": traffic_for(workload_key),
+            "bytes_per_launch": move_bytes,
+            "avg_launch_ms": move_avg_s * 1e3,
+            "launches": n_move,
+            "model": "8 B/agent (stepsUntilMove read + write) + 4 B/relocated agent, per movement launch",
+            "workload_key": workload_key,
         },
-        "step_pipeline": {
-            "pipeline_ms_per_step": pipe_ms / max(1, n_pipe),
-            "midnight_and_gaps_ms_per_step": (elapsed * 1e3 - pipe_ms) / K,
-            "warmup_movement_ms_per_step": wmove_ms / max(1, wm),
-            "warmup_infection_ms_per_step": winfect_ms / max(1, wi),
-            "movers_evaluated_per_step": evaluated / K,
-            "relocated_per_step": relocated / K,
-            "algorithmic_bytes_per_step": step_bytes_total / K,
+        "movement_design": {
+            "bytes_per_launch": design_bytes,
+            "achieved_GBs": design_bytes / move_avg_s / 1e9 if move_avg_s > 0 else 0.0,
+            "frac": design_bytes / move_avg_s / 1e9 / HBM_PEAK_GBS if move_avg_s > 0 else 0.0,
+            "model": "2 B/agent wake scan + 62 B/evaluated agent + 48 B/relocation",
+            "movers_evaluated_per_step": evaluated / max(1, K),
+            "relocated_per_step": relocated / max(1, K),
         },
-        "final_day_stats": last[:12],
+        "window_day_stats": st[:12],
+        "full_day": full_day,
         "ensemble": ensemble,
         "gradient": gradient,
         "cpu_baseline": cpu,

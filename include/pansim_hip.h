@@ -315,10 +315,13 @@ int pansim_infection_dump_collect(pansim_ctx* ctx, uint32_t* new_agent, uint32_t
  *   work; pansim_timing_summary returns the number of timed steps (movement
  *   launches) and their summed device time as n_move / move_ms, and zero
  *   infection calls.
+ * PANSIM_TIMING_MOVEMENT: as PANSIM_TIMING_KERNELS for the movement launches
+ *   only (half the events; the infection launches are not timed).
  * pansim_enable_timing resets the record. */
 #define PANSIM_TIMING_OFF 0u
 #define PANSIM_TIMING_KERNELS 1u
 #define PANSIM_TIMING_PIPELINE 2u
+#define PANSIM_TIMING_MOVEMENT 3u
 int pansim_enable_timing(pansim_ctx* ctx, uint32_t mode);
 int pansim_timing_summary(pansim_ctx* ctx, uint32_t* n_move, double* move_ms, uint32_t* n_infect, double* infect_ms);
 int pansim_last_kernel_ms(pansim_ctx* ctx, float* move_ms, float* infect_ms);

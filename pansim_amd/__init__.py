@@ -233,11 +233,13 @@ class Simulation:
             raise PanSimError(rc, lib().pansim_last_error(self.ctx).decode())
         return begin, count, out[:total]
 
-    def enable_timing(self, on: bool = True, pipeline: bool = False):
+    def enable_timing(self, on: bool = True, pipeline: bool = False, movement_only: bool = False):
         """HIP-event timing of movement and infection: one event pair per call
-        (kernel mode) or, with pipeline=True, one pair around each run of
-        movement + infection steps between other calls (a day's steps)."""
-        self._check_ctx(lib().pansim_enable_timing(self.ctx, (2 if pipeline else 1) if on else 0))
+        (kernel mode), only per movement call (movement_only=True) or, with
+        pipeline=True, one pair around each run of movement + infection steps
+        between other calls (a day's steps)."""
+        mode = 2 if pipeline else 3 if movement_only else 1
+        self._check_ctx(lib().pansim_enable_timing(self.ctx, mode if on else 0))
 
     def timing_summary(self):
         """(n_movement_calls, movement_ms, n_infection_calls, infection_ms) since enable_timing;

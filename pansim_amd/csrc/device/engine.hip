@@ -490,7 +490,7 @@ int pansim_sync(pansim_ctx* c) {
 int pansim_enable_timing(pansim_ctx* c, uint32_t on) {
     if (!c) return PANSIM_E_ARG;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    if (on > PANSIM_TIMING_PIPELINE) return fail(c, PANSIM_E_ARG, "enable_timing: unknown mode");
+    if (on > PANSIM_TIMING_MOVEMENT) return fail(c, PANSIM_E_ARG, "enable_timing: unknown mode");
     c->timing = on;
     c->moveN = c->infectN = 0;
     c->pipeOpen = false;
@@ -605,7 +605,8 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     if (p->timestamp - m.stepOfDay != c->wakeDay)
         return fail(c, PANSIM_E_ARG, "movement_step: pansim_plan_locations must run at the start of every day");
     c->pendPar ^= 1u;
-    if (c->timing == PANSIM_TIMING_KERNELS && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
+    const bool perLaunch = c->timing == PANSIM_TIMING_KERNELS || c->timing == PANSIM_TIMING_MOVEMENT;
+    if (perLaunch && record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
     if (c->timing == PANSIM_TIMING_PIPELINE && !c->pipeOpen) {
         if (record_event(c, c->moveEv, 2 * c->moveN)) return PANSIM_E_DEVICE;
         c->pipeOpen = true;
@@ -613,7 +614,7 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
     c->resolvePending = true;
     c->resolvePar = m.pendPar;
-    if (c->timing == PANSIM_TIMING_KERNELS) {
+    if (perLaunch) {
         if (record_event(c, c->moveEv, 2 * c->moveN + 1)) return PANSIM_E_DEVICE;
         c->moveN++;
     }
