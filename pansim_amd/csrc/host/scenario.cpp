@@ -91,6 +91,7 @@ std::vector<OptSpec> option_table(Options& o) {
         {"", "synthAgents", OptSpec::I, &o.synthAgents},
         {"", "synthSeed", OptSpec::U64, &o.synthSeed},
         {"", "synthInfected", OptSpec::D, &o.synthInfected},
+        {"", "synthJitter", OptSpec::U, &o.synthJitter},
         {"", "movementPolicy", OptSpec::S, &o.movementPolicy},
         {"", "testingPolicy", OptSpec::S, &o.testingPolicy},
         {"", "closurePolicy", OptSpec::S, &o.closurePolicy},

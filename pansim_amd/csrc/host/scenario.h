@@ -115,6 +115,10 @@ struct Options {
     int synthAgents = -1;            // --synthAgents: synthetic household population of this size
     uint64_t synthSeed = 1;          // --synthSeed: generator seed for --synthAgents
     double synthInfected = 0.001;    // --synthInfected: initial fraction in the first infected state
+    // --synthJitter: schedule jitter of the synthetic population in minutes:
+    // every agent follows its type's schedule shifted by one of
+    // -J, -J+dt, ..., +J (uniform), stored as shifted agent-type variants
+    unsigned synthJitter = 30;
     // --movementPolicy real|dummy|none: RealMovement, DummyMovement or
     // NoMovement (movementPolicies.h:9-66, 1339-1752), which the reference
     // selects at build time (CMakeLists.txt:76-85)

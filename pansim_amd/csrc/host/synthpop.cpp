@@ -19,6 +19,14 @@
 //   schools      300-800 pupils in classrooms of 25
 //   workplaces   lognormal sizes (mean 15, sigma 1.2, capped at 2000)
 //   shops / leisure / hospitals  one per 200 / 500 / 20000 agents
+//   schedule jitter  each agent follows its type's schedule shifted by a
+//                uniform multiple of dt in [-J, +J] (--synthJitter J,
+//                default 30 min): the shifted schedules are extra agent
+//                types in the reference's schedule format (every event's
+//                start and end moved, durations kept), appended after the
+//                file's types; the afternoon-shift type (ID 7) is not
+//                shifted, because the reference keys its curfew exception
+//                on that type's index (movementPolicies.h:273, 829)
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -160,6 +168,59 @@ void build_synthetic(Scenario& sc) {
             --left;
         }
     }
+    // schedule jitter: agent-type variants with every event moved by a
+    // multiple of dt in [-J, +J]; variant v of base type b sits at index
+    // nBase + (b' * nShift + v), b' counting the shifted base types
+    const unsigned dt = sc.opt.deltat ? sc.opt.deltat : 10;
+    const int J = (int)(sc.opt.synthJitter / dt) * (int)dt;
+    std::vector<int> shifts;
+    for (int d = -J; d <= J; d += (int)dt)
+        if (d != 0) shifts.push_back(d);
+    const unsigned nBase = sc.sched.nTypes;
+    const unsigned nightIdx = 6;  // agent type ID 7
+    std::vector<int> firstVariant(nBase, -1);
+    if (!shifts.empty()) {
+        std::vector<uint32_t> baseOff = sc.sched.eventOffset;
+        std::vector<Event> baseEv = sc.sched.events;
+        unsigned nTypes = nBase;
+        for (unsigned b = 0; b < nBase; ++b)
+            if (b != nightIdx) {
+                firstVariant[b] = (int)nTypes;
+                nTypes += (unsigned)shifts.size();
+            }
+        if (nTypes > 255) throw InitError(ERR_BAD_DATA, "synthJitter: too many agent-type variants");
+        auto moved = [](uint8_t h, uint8_t m, int d) {
+            if (h == 255) return std::make_pair(h, m);  // undefined time (-1)
+            int t = std::min(1440, std::max(0, (int)h * 60 + (int)m + d));
+            return std::make_pair((uint8_t)(t / 60), (uint8_t)(t % 60));
+        };
+        std::vector<uint32_t> off(nTypes * 28 + 1, 0);
+        std::vector<Event> ev = baseEv;
+        for (unsigned s = 0; s < nBase * 28; ++s) off[s + 1] = baseOff[s + 1];
+        for (unsigned b = 0; b < nBase; ++b) {
+            if (firstVariant[b] < 0) continue;
+            for (size_t v = 0; v < shifts.size(); ++v) {
+                unsigned t = (unsigned)firstVariant[b] + (unsigned)v;
+                for (unsigned slot = 0; slot < 28; ++slot) {
+                    for (uint32_t k = baseOff[b * 28 + slot]; k < baseOff[b * 28 + slot + 1]; ++k) {
+                        Event e = baseEv[k];
+                        auto st = moved(e.start_h, e.start_m, shifts[v]);
+                        auto en = moved(e.end_h, e.end_m, shifts[v]);
+                        e.start_h = st.first;
+                        e.start_m = st.second;
+                        e.end_h = en.first;
+                        e.end_m = en.second;
+                        ev.push_back(e);
+                    }
+                    off[t * 28 + slot + 1] = (uint32_t)ev.size();
+                }
+            }
+        }
+        sc.sched.nTypes = nTypes;
+        sc.sched.eventOffset = off;
+        sc.sched.events = ev;
+    }
+
     // cemetery
     L.cemetery = (uint32_t)L.type.size();
     L.type.push_back(lt.cemeteryType);
@@ -215,6 +276,11 @@ void build_synthetic(Scenario& sc) {
         A.state[i] = (uint8_t)(rs.unit() < sc.opt.synthInfected ? sc.st.firstInfected : susState);
         int tid = typeOf[i];
         A.typeIdx[i] = (uint8_t)(tid - 1);  // agentTypes.json IDs 1..9 in file order
+        if (!shifts.empty() && firstVariant[tid - 1] >= 0) {
+            Stream rj(seed, i, 7, PH_SYNTH);
+            uint32_t pick = rj.uint_below((uint32_t)shifts.size() + 1);  // 0 = unshifted
+            if (pick > 0) A.typeIdx[i] = (uint8_t)(firstVariant[tid - 1] + (int)pick - 1);
+        }
         locs.clear();
         locs.push_back({(uint8_t)lt.home, homeOf[i]});
         locs.push_back({(uint8_t)lt.hospital, hospitals.first + rs.uint_below(hospitals.n)});

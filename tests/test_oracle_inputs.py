@@ -177,3 +177,47 @@ def test_dummy_agent_locations(dummy):
         b, e = int(off[i]), int(off[i + 1])
         homes = [int(locs[k]) for k in range(b, e) if ptypes[k] == 2]
         assert loc0[i] == (homes[0] if homes else 0)
+
+
+def test_synthetic_schedule_jitter():
+    """--synthJitter 30 (the default): the synthetic population's agent types
+    are the file's nine, then, for every type but the afternoon shift (ID 7,
+    whose index the reference's curfew exception tests), six variants whose
+    events start and end -30, -20, -10, +10, +20 and +30 minutes from the
+    base type's (clamped to the day), with the same location types, chances
+    and durations; every agent of a shifted type family follows one of the
+    seven schedules with equal chance."""
+    o = OracleSim(pansim_amd.with_data_dir(pansim_amd.synthetic_args(30000, 1, 0, 1)))
+    try:
+        off, ty, ch, tm = o.schedule()
+        at = o.field("agentType")
+    finally:
+        o.close()
+    base_types = json.load(open(os.path.join(CFG, "agentTypes.json")))["types"]
+    nb = len(base_types)
+    shifts = [-30, -20, -10, 10, 20, 30]
+    assert len(off) == 28 * (nb + 6 * (nb - 1)) + 1
+    mins = lambda h, m: int(h) * 60 + int(m)
+    variant = nb
+    for b in range(nb):
+        if base_types[b]["ID"] == 7:
+            continue
+        for d in shifts:
+            for slot in range(28):
+                e0, e1 = off[28 * b + slot], off[28 * b + slot + 1]
+                v0, v1 = off[28 * variant + slot], off[28 * variant + slot + 1]
+                assert v1 - v0 == e1 - e0
+                for k in range(e1 - e0):
+                    x, y = e0 + k, v0 + k
+                    assert ty[x] == ty[y] and ch[x] == ch[y] and tuple(tm[x][4:]) == tuple(tm[y][4:])
+                    for p in (0, 2):
+                        want = min(1440, max(0, mins(*tm[x][p:p + 2]) + d))
+                        assert mins(*tm[y][p:p + 2]) == want, (b, d, slot, k)
+            variant += 1
+    # agents of a shifted family spread evenly over the seven schedules
+    worker = [t["ID"] for t in base_types].index(6)
+    fam = np.concatenate([[worker], nb + 6 * (worker - (1 if worker > 6 else 0)) + np.arange(6)])
+    counts = np.array([(at == t).sum() for t in fam])
+    assert counts.sum() > 5000
+    exp = counts.sum() / 7
+    assert np.all(np.abs(counts - exp) <= 4.5 * np.sqrt(exp)), counts
