@@ -217,6 +217,25 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
             g_create_error = "pansim_create: a susceptible state with non-zero infectiousness is not supported";
             return PANSIM_E_UNSUPPORTED;
         }
+    if (t->first_infected >= d->n_states || t->non_covid_dead >= d->n_states || t->dead_state >= d->n_states) {
+        g_create_error = "pansim_create: first infected, non-COVID dead or dead state out of range";
+        return PANSIM_E_ARG;
+    }
+    // The per-location infectiousness sums are kept exactly in 2^-32 fixed
+    // point.  An agent's infectiousness is the float product of its state's
+    // value and its variant's multiplier; a float of at least 2^-9 is a
+    // multiple of 2^-32, and below 2^31 / n_agents no sum can leave the
+    // 64-bit range.  Other values would be truncated: refuse them.
+    for (uint32_t s = 0; s < d->n_states; ++s)
+        for (uint32_t v = 0; v < d->n_variants; ++v) {
+            const float x = t->state_infectious[s] * t->variant_multiplier[v];
+            if (x == 0.0f) continue;
+            if (!(x >= 0x1p-9f) || !((double)x * (double)d->n_agents < 2147483648.0)) {
+                g_create_error = "pansim_create: an agent infectiousness (state value times variant multiplier) "
+                                 "outside [2^-9, 2^31 / n_agents) is not supported (exact fixed-point sums)";
+                return PANSIM_E_UNSUPPORTED;
+            }
+        }
     if (t->state_infectious[t->first_infected] != 0.0f) {
         g_create_error = "pansim_create: non-zero infectiousness of the first infected state is not supported";
         return PANSIM_E_UNSUPPORTED;
