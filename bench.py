@@ -80,7 +80,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all host threads)")
     ap.add_argument("--ensemble-replicas", type=int, default=64,
                     help="replicas of the batched-ensemble measurement (0 skips it)")
-    ap.add_argument("--ensemble-steps", type=int, default=288)
+    ap.add_argument("--ensemble-days", type=int, default=7)
     ap.add_argument("--gradient-replicas", type=int, default=32,
                     help="replicas of the gradient measurement (0 skips it)")
     ap.add_argument("--gradient-agents", type=int, default=1_000_000)
@@ -210,24 +210,27 @@ def ensemble_args(n_rep, weeks):
 
 
 def ensemble_measure(a):
-    """The same replicas stepped as one batch (one launch per kernel per step
-    for all of them) and one after the other with the plain path; day 0 is
-    warmup, then --ensemble-steps steps are timed in each mode."""
+    """The same replicas stepped as one batch (every day one replay of the
+    captured day graph: per step one movement and one infection launch for all
+    replicas, the midnight blocks between the replays) and one after the
+    other with the plain path; day 0 is warmup, then --ensemble-days whole
+    days are timed in each mode."""
     import pansim_amd
 
     reps, total = ensemble_args(a.ensemble_replicas, a.weeks)
-    K = a.ensemble_steps
+    D = a.ensemble_days
+    K = D * 144
     t0 = time.perf_counter()
     ens = pansim_amd.Ensemble(reps)
     log(f"ensemble: {a.ensemble_replicas} replicas, {total} agents built in {time.perf_counter() - t0:.1f}s")
     ens.steps(144)
     ens.sync()
     t1 = time.perf_counter()
-    ens.steps(K)
+    ens.days(D)
     ens.sync()
     batched = time.perf_counter() - t1
     ens.close()
-    log(f"ensemble: batched {K} steps in {batched:.3f}s")
+    log(f"ensemble: batched {D} days ({K} steps) in {batched:.3f}s")
     looped = 0.0
     for args in reps:
         sim = pansim_amd.Simulation(args)
@@ -246,6 +249,7 @@ def ensemble_measure(a):
         "agents": total,
         "steps": K,
         "warmup": 144,
+        "mode": "batched: one hipGraph replay per simulated day; looped: the plain path per replica",
         "batched_agent_steps_per_s": total * K / batched,
         "looped_agent_steps_per_s": total * K / looped,
         "speedup": looped / batched,

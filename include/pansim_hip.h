@@ -350,6 +350,12 @@ int pansim_device_philox(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_
  * dJ/dk, out[2 + t] = dJ/dm_t; n >= 258. */
 int pansim_set_location_type_multipliers(pansim_ctx* ctx, const float* m, uint32_t n);
 int pansim_hazard_enable(pansim_ctx* ctx, uint32_t on);
+/* Finite-difference probes along the same trajectory (0 <= h < 1; h = 0
+ * switches them off): with them, out[258..261] of pansim_hazard_read (n >=
+ * 262) hold J(k (1+h)) - J, J(k (1-h)) - J, and the same for the multiplier
+ * of location type `type`; (out[258] - out[259]) / (2 h k) is a central
+ * difference of dJ/dk. */
+int pansim_hazard_set_probe(pansim_ctx* ctx, double h, uint32_t type);
 int pansim_hazard_read(pansim_ctx* ctx, double* out, uint32_t n, uint32_t reset);
 
 /* ---- Replica batches --------------------------------------------------
@@ -371,6 +377,22 @@ const char* pansim_batch_error(const pansim_batch* batch);
 int pansim_batch_set_day(pansim_batch* batch, const pansim_step_params* params, const uint32_t* variant_masks,
                          const float* k, const float* seasonality);
 int pansim_batch_step(pansim_batch* batch, uint32_t timestamp);
+/* The whole day that starts at `timestamp` (the first step of the day given
+ * to pansim_batch_set_day) as one graph launch: the day's steps are captured
+ * once as a hipGraph of their movement and infection launches and replayed
+ * every day (captured again when hazard accumulation or the pending-write
+ * parity changes).  Equivalent to steps_per_day calls of pansim_batch_step. */
+int pansim_batch_run_day(pansim_batch* batch, uint32_t timestamp);
+/* Tile sizes of the batch kernels: agents per thread of the movement tile
+ * (8, 16 or 32; its LDS list of movers holds 256 times as many entries) and
+ * of the infection tile (4 or 8).  By default an autotuner picks them: the
+ * eager pansim_batch_step calls before the first pansim_batch_run_day cycle
+ * through the six pairs, timed with HIP events, and the fastest pair on
+ * average is used from then on (the results do not depend on the sizes).
+ * pansim_batch_set_tiles(b, 0, 0) restarts the tuning; other values fix the
+ * sizes.  pansim_batch_tiles reports the pair in use (0, 0 while tuning). */
+int pansim_batch_set_tiles(pansim_batch* batch, uint32_t move_items, uint32_t infect_items);
+int pansim_batch_tiles(pansim_batch* batch, uint32_t* move_items, uint32_t* infect_items);
 int pansim_batch_sync(pansim_batch* batch);
 
 #ifdef __cplusplus

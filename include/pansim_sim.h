@@ -71,6 +71,12 @@ const char* pansim_ensemble_create_error(void);
 const char* pansim_ensemble_error(const pansim_ensemble* ens);
 void pansim_ensemble_destroy(pansim_ensemble* ens);
 int pansim_ensemble_steps(pansim_ensemble* ens, uint32_t n);
+/* n whole days from a midnight, each one replay of the captured day graph
+ * (pansim_batch_run_day); the results equal those of pansim_ensemble_steps. */
+int pansim_ensemble_days(pansim_ensemble* ens, uint32_t n);
+/* The replica batch of the engine ABI under the ensemble (for its tile-size
+ * controls, pansim_batch_set_tiles / pansim_batch_tiles). */
+struct pansim_batch* pansim_ensemble_batch(pansim_ensemble* ens);
 int pansim_ensemble_run(pansim_ensemble* ens);
 int pansim_ensemble_sync(pansim_ensemble* ens);
 uint32_t pansim_ensemble_size(const pansim_ensemble* ens);

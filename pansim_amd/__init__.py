@@ -96,6 +96,13 @@ def lib() -> ctypes.CDLL:
     L.pansim_ensemble_error.restype = ctypes.c_char_p
     L.pansim_ensemble_destroy.argtypes = [ctypes.c_void_p]
     L.pansim_ensemble_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_ensemble_days.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    L.pansim_ensemble_batch.argtypes = [ctypes.c_void_p]
+    L.pansim_ensemble_batch.restype = ctypes.c_void_p
+    L.pansim_batch_set_tiles.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    L.pansim_batch_tiles.argtypes = [ctypes.c_void_p, c_u32p, c_u32p]
+    L.pansim_batch_error.argtypes = [ctypes.c_void_p]
+    L.pansim_batch_error.restype = ctypes.c_char_p
     L.pansim_ensemble_run.argtypes = [ctypes.c_void_p]
     L.pansim_ensemble_sync.argtypes = [ctypes.c_void_p]
     L.pansim_ensemble_size.argtypes = [ctypes.c_void_p]
@@ -113,6 +120,7 @@ def lib() -> ctypes.CDLL:
     L.pansim_set_location_type_multipliers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]
     L.pansim_hazard_enable.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     L.pansim_hazard_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32]
+    L.pansim_hazard_set_probe.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32]
     _lib = L
     return L
 
@@ -274,10 +282,20 @@ class Simulation:
 
     def hazard(self, reset: bool = False):
         """(J, dJ/dk, dJ/dm[256]): expected infections and their gradient so far."""
-        out = np.zeros(258)
-        self._check_ctx(lib().pansim_hazard_read(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 258,
+        out = self.hazard_with_probes(reset)
+        return out[0], out[1], out[2:258]
+
+    def hazard_set_probe(self, h: float, loc_type: int):
+        """Finite-difference probes along the trajectory: J at k(1 +- h) and at
+        the multiplier of loc_type scaled by (1 +- h) (h = 0: off)."""
+        self._check_ctx(lib().pansim_hazard_set_probe(self.ctx, h, loc_type))
+
+    def hazard_with_probes(self, reset: bool = False) -> np.ndarray:
+        """[J, dJ/dk, dJ/dm[256], J(k+) - J, J(k-) - J, J(m_p+) - J, J(m_p-) - J]."""
+        out = np.zeros(262)
+        self._check_ctx(lib().pansim_hazard_read(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 262,
                                                  1 if reset else 0))
-        return out[0], out[1], out[2:]
+        return out
 
     def location_sums(self, variant: int) -> np.ndarray:
         out = np.zeros(self.n_locations, dtype=np.int64)
@@ -325,6 +343,23 @@ class Ensemble:
 
     def steps(self, n: int):
         self._check(lib().pansim_ensemble_steps(self._h, n))
+
+    def set_tiles(self, move_items: int = 0, infect_items: int = 0):
+        """Fixes the batch kernels' agents per thread (movement 8/16/32,
+        infection 4/8), or (0, 0): let the autotuner pick them."""
+        b = lib().pansim_ensemble_batch(self._h)
+        if lib().pansim_batch_set_tiles(b, move_items, infect_items) != 0:
+            raise PanSimError(-1, lib().pansim_batch_error(b).decode())
+
+    def tiles(self):
+        """(movement, infection) agents per thread in use; (0, 0) while tuning."""
+        m, i = ctypes.c_uint32(), ctypes.c_uint32()
+        lib().pansim_batch_tiles(lib().pansim_ensemble_batch(self._h), ctypes.byref(m), ctypes.byref(i))
+        return m.value, i.value
+
+    def days(self, n: int):
+        """n whole days from a midnight, each one replay of the captured day graph."""
+        self._check(lib().pansim_ensemble_days(self._h, n))
 
     def run(self):
         self._check(lib().pansim_ensemble_run(self._h))
@@ -377,6 +412,8 @@ class EnsembleReplica:
     set_location_type_multipliers = Simulation.set_location_type_multipliers
     hazard_enable = Simulation.hazard_enable
     hazard = Simulation.hazard
+    hazard_set_probe = Simulation.hazard_set_probe
+    hazard_with_probes = Simulation.hazard_with_probes
     _check_ctx = Simulation._check_ctx
 
 

@@ -52,6 +52,8 @@ struct pansim_ctx {
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
     bool hazardOn = false;
+    float probeH = 0.0f;      // finite-difference probes of the hazard (0: none)
+    uint32_t probeType = 0;
     // per-variant infected counts of the last statistics pass, valid while no
     // state-changing launch has run since (stateEpoch counts those)
     uint64_t stateEpoch = 1, varEpoch = 0;
@@ -75,6 +77,12 @@ struct pansim_ctx {
 // A batch of replicas stepped together (include/pansim_hip.h): the member
 // contexts keep their arrays; the batch owns the per-replica argument tables
 // and the stream every member works on while it is batched.
+// tile-size candidates of the replica batches' autotuner: agents per thread
+// of the movement tile (its LDS mover list holds 256 x that many offsets)
+// and of the infection tile
+static constexpr uint32_t kMoveItems[3] = {8, 16, 32};
+static constexpr uint32_t kInfectItems[2] = {4, 8};
+
 struct pansim_batch {
     std::vector<pansim_ctx*> reps;
     std::vector<hipStream_t> ownStreams;  // members' streams, restored on destroy
@@ -82,13 +90,32 @@ struct pansim_batch {
     std::string err;
     uint32_t dt = 10;
     uint32_t pendPar = 0;
-    uint32_t moveBlocks = 0, infBlocks = 0;
     ps::dev::MoveArgs* dMove = nullptr;
     ps::dev::InfectArgs* dInf = nullptr;
-    uint32_t* dTables = nullptr;  // moveRep | moveFirst | infRep | infFirst
+    // block -> replica tables for every tile size the autotuner may pick:
+    // per movement size moveRep | moveFirst, per infection size infRep | infFirst
+    uint32_t* dTables = nullptr;
+    uint32_t moveOff[3] = {0, 0, 0}, moveBlocks[3] = {0, 0, 0};
+    uint32_t infOff[2] = {0, 0}, infBlocks[2] = {0, 0};
+    // tile sizes in use (indices into the candidate lists); -1 = still tuning
+    int moveCfg = -1, infCfg = -1;
+    bool autotune = true;
+    struct TuneSample {
+        int move, inf;
+        hipEvent_t e0, e1;
+    };
+    std::vector<TuneSample> samples;
     ps::dev::MoveArgs* hMove = nullptr;  // pinned staging of the per-day arguments
     ps::dev::InfectArgs* hInf = nullptr;
+    uint32_t* dDayStart = nullptr;  // the day's first timestamp, read by the batch kernels
+    uint32_t* hDayStart = nullptr;  // its pinned staging
     bool haveDay = false;
+    // one simulated day of batch steps captured as a graph (pansim_batch_run_day);
+    // it depends on the hazard choice and the pending-write parity at the day start
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graphHazard = -1;
+    uint32_t graphPar = 0;
 };
 
 namespace {
@@ -466,6 +493,9 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(dalloc(c, &c->S.pendList, c->S.pendCap));
     TRY(dalloc(c, &c->S.pendCount, 2));
     HIP_OR_FAIL(c, hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), c->stream));
+    TRY(dalloc(c, &c->S.rareList, (size_t)d->n_agents + 16u));
+    TRY(dalloc(c, &c->S.rareCount, 2));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->S.rareCount, 0, 2 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &c->counters, S_COUNT + 16));
     if (hipHostMalloc((void**)&c->hostCounters, (S_COUNT + 16) * sizeof(unsigned long long), hipHostMallocDefault) !=
         hipSuccess) {
@@ -607,6 +637,8 @@ static InfectArgs infect_args(pansim_ctx* c, uint32_t timestamp, uint32_t mask, 
     a.dt = (float)dt;
     a.typeMul = c->typeMul;
     a.hazard = c->hazardOn ? c->hazard : nullptr;
+    a.probeH = c->probeH;
+    a.probeType = c->probeType;
     a.dumpList = c->dumpOn ? c->dumpNew : nullptr;
     a.dumpCount = c->dumpCnt;
     a.resolve = 0;
@@ -631,6 +663,7 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
         c->pipeOpen = true;
     }
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
+    hipLaunchKernelGGL(k_move_rare, dim3(MOVE_RARE_BLOCKS), dim3(MOVE_BLOCK), 0, c->stream, m);
     c->resolvePending = true;
     c->resolvePar = m.pendPar;
     if (perLaunch) {
@@ -1300,11 +1333,19 @@ int pansim_hazard_enable(pansim_ctx* c, uint32_t on) {
     return 0;
 }
 
+int pansim_hazard_set_probe(pansim_ctx* c, double h, uint32_t type) {
+    if (!c || !(h >= 0.0 && h < 1.0) || type >= HAZARD_TYPES) return PANSIM_E_ARG;
+    c->probeH = (float)h;
+    c->probeType = type;
+    return 0;
+}
+
 int pansim_hazard_read(pansim_ctx* c, double* out, uint32_t n, uint32_t reset) {
-    if (!c || !out || n < HAZARD_SLOTS) return PANSIM_E_ARG;
+    if (!c || !out || n < HAZARD_PROBE0) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    HIP_OR_FAIL(c, hipMemcpy(out, c->hazard, HAZARD_SLOTS * sizeof(double), hipMemcpyDeviceToHost));
+    const uint32_t m = n < HAZARD_SLOTS ? HAZARD_PROBE0 : HAZARD_SLOTS;
+    HIP_OR_FAIL(c, hipMemcpy(out, c->hazard, m * sizeof(double), hipMemcpyDeviceToHost));
     if (reset) HIP_OR_FAIL(c, hipMemset(c->hazard, 0, HAZARD_SLOTS * sizeof(double)));
     return 0;
 }
@@ -1319,6 +1360,14 @@ void pansim_batch_destroy(pansim_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (size_t r = 0; r < b->reps.size() && r < b->ownStreams.size(); ++r) b->reps[r]->stream = b->ownStreams[r];
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& x : b->samples) {
+        (void)hipEventDestroy(x.e0);
+        (void)hipEventDestroy(x.e1);
+    }
+    if (b->dDayStart) (void)hipFree(b->dDayStart);
+    if (b->hDayStart) (void)hipHostFree(b->hDayStart);
     if (b->dMove) (void)hipFree(b->dMove);
     if (b->dInf) (void)hipFree(b->dInf);
     if (b->dTables) (void)hipFree(b->dTables);
@@ -1353,7 +1402,6 @@ int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_
     b->dt = replicas[0]->dt;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipStreamCreate failed");
-    std::vector<uint32_t> moveRep, moveFirst(n), infRep, infFirst(n);
     for (uint32_t r = 0; r < n; ++r) {
         pansim_ctx* c = replicas[r];
         if (flush_resolve(c) != 0 || hipStreamSynchronize(c->stream) != hipSuccess)
@@ -1361,29 +1409,37 @@ int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_
         b->reps.push_back(c);
         b->ownStreams.push_back(c->stream);
         c->stream = b->stream;
-        moveFirst[r] = (uint32_t)moveRep.size();
-        for (uint32_t t = 0; t < blocks_for(c->A.n, MOVE_TILE); ++t) moveRep.push_back(r);
-        infFirst[r] = (uint32_t)infRep.size();
-        for (uint32_t t = 0; t < blocks_for(c->A.n, INFECT_TILE); ++t) infRep.push_back(r);
         // at rest neither pending-write counter holds anything: restart all at parity 0
-        if (hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess)
+        if (hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess ||
+            hipMemsetAsync(c->S.rareCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess)
             return bail(PANSIM_E_DEVICE, "pansim_batch_create: memset failed");
         c->pendPar = 0;
     }
-    b->moveBlocks = (uint32_t)moveRep.size();
-    b->infBlocks = (uint32_t)infRep.size();
-    const size_t nt = moveRep.size() + n + infRep.size() + n;
+    // block -> replica tables of every candidate tile size
+    std::vector<uint32_t> tables;
+    auto addTables = [&](uint32_t tile, uint32_t& off, uint32_t& blocks) {
+        std::vector<uint32_t> rep, first(n);
+        for (uint32_t r = 0; r < n; ++r) {
+            first[r] = (uint32_t)rep.size();
+            for (uint32_t t = 0; t < blocks_for(replicas[r]->A.n, tile); ++t) rep.push_back(r);
+        }
+        off = (uint32_t)tables.size();
+        blocks = (uint32_t)rep.size();
+        tables.insert(tables.end(), rep.begin(), rep.end());
+        tables.insert(tables.end(), first.begin(), first.end());
+    };
+    for (int k = 0; k < 3; ++k) addTables(MOVE_BLOCK * kMoveItems[k], b->moveOff[k], b->moveBlocks[k]);
+    for (int k = 0; k < 2; ++k) addTables(INFECT_BLOCK * kInfectItems[k], b->infOff[k], b->infBlocks[k]);
+    const size_t nt = tables.size();
     if (hipMalloc(&b->dMove, n * sizeof(MoveArgs)) != hipSuccess || hipMalloc(&b->dInf, n * sizeof(InfectArgs)) != hipSuccess ||
         hipMalloc(&b->dTables, nt * sizeof(uint32_t)) != hipSuccess)
         return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipMalloc failed");
+    if (hipMalloc(&b->dDayStart, sizeof(uint32_t)) != hipSuccess)
+        return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipMalloc failed");
     if (hipHostMalloc((void**)&b->hMove, n * sizeof(MoveArgs), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&b->hInf, n * sizeof(InfectArgs), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&b->hInf, n * sizeof(InfectArgs), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->hDayStart, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return bail(PANSIM_E_DEVICE, "pansim_batch_create: hipHostMalloc failed");
-    std::vector<uint32_t> tables;
-    tables.insert(tables.end(), moveRep.begin(), moveRep.end());
-    tables.insert(tables.end(), moveFirst.begin(), moveFirst.end());
-    tables.insert(tables.end(), infRep.begin(), infRep.end());
-    tables.insert(tables.end(), infFirst.begin(), infFirst.end());
     if (hipMemcpy(b->dTables, tables.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess)
         return bail(PANSIM_E_DEVICE, "pansim_batch_create: table upload failed");
@@ -1403,10 +1459,89 @@ int pansim_batch_set_day(pansim_batch* b, const pansim_step_params* params, cons
         b->hMove[r] = move_args(c, &params[r]);
         b->hInf[r] = infect_args(c, params[r].timestamp, variant_masks[r], k[r], seasonality[r], b->dt);
     }
+    const uint32_t spd = 1440u / b->dt;
+    *b->hDayStart = params[0].timestamp - params[0].timestamp % spd;
     if (hipMemcpyAsync(b->dMove, b->hMove, n * sizeof(MoveArgs), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->dInf, b->hInf, n * sizeof(InfectArgs), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        hipMemcpyAsync(b->dInf, b->hInf, n * sizeof(InfectArgs), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->dDayStart, b->hDayStart, sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return batch_fail(b, PANSIM_E_DEVICE, "pansim_batch_set_day: upload failed");
     b->haveDay = true;
+    return 0;
+}
+
+static bool batch_hazard(const pansim_batch* b) {
+    for (const pansim_ctx* c : b->reps)
+        if (c->hazardOn) return true;
+    return false;
+}
+
+// One batch step: the movement launch of every replica, then the infection
+// launch (which also applies the movement's deferred location writes), with
+// the tile sizes of candidates mc (movement) and ic (infection).
+extern "C++" {
+template <uint32_t MI, uint32_t II>
+static void launch_batch_sized(pansim_batch* b, const BatchArgs& a, uint32_t mc, uint32_t ic, bool hazard) {
+    hipLaunchKernelGGL(k_move_batch<MI>, dim3(b->moveBlocks[mc]), dim3(MOVE_BLOCK), 0, b->stream, a);
+    hipLaunchKernelGGL(k_move_rare_batch, dim3(a.nRep * MOVE_RARE_BATCH_BLOCKS), dim3(MOVE_BLOCK), 0, b->stream, a);
+    if (hazard) hipLaunchKernelGGL((k_infect_batch<true, II>), dim3(b->infBlocks[ic]), dim3(INFECT_BLOCK), 0, b->stream, a);
+    else hipLaunchKernelGGL((k_infect_batch<false, II>), dim3(b->infBlocks[ic]), dim3(INFECT_BLOCK), 0, b->stream, a);
+}
+}
+
+static void launch_batch_step(pansim_batch* b, uint32_t stepOfDay, uint32_t pendPar, bool hazard, int mc, int ic) {
+    const uint32_t n = (uint32_t)b->reps.size();
+    BatchArgs a;
+    a.move = b->dMove;
+    a.infect = b->dInf;
+    a.moveRep = b->dTables + b->moveOff[mc];
+    a.moveFirst = a.moveRep + b->moveBlocks[mc];
+    a.infRep = b->dTables + b->infOff[ic];
+    a.infFirst = a.infRep + b->infBlocks[ic];
+    a.dayStart = b->dDayStart;
+    a.stepOfDay = stepOfDay;
+    a.pendPar = pendPar;
+    a.nRep = n;
+    a.infBlocks = b->infBlocks[ic];
+    switch (mc * 2 + ic) {
+    case 0: launch_batch_sized<kMoveItems[0], kInfectItems[0]>(b, a, 0, 0, hazard); break;
+    case 1: launch_batch_sized<kMoveItems[0], kInfectItems[1]>(b, a, 0, 1, hazard); break;
+    case 2: launch_batch_sized<kMoveItems[1], kInfectItems[0]>(b, a, 1, 0, hazard); break;
+    case 3: launch_batch_sized<kMoveItems[1], kInfectItems[1]>(b, a, 1, 1, hazard); break;
+    case 4: launch_batch_sized<kMoveItems[2], kInfectItems[0]>(b, a, 2, 0, hazard); break;
+    default: launch_batch_sized<kMoveItems[2], kInfectItems[1]>(b, a, 2, 1, hazard); break;
+    }
+}
+
+// The autotuner: while no tile sizes are chosen, eager batch steps cycle
+// through the six candidate pairs with an event pair around each step (the
+// results do not depend on the tile sizes: every agent's work is its own and
+// the per-location updates are integer atomics); pansim_batch_run_day then
+// picks the pair with the least mean step time before it captures the day.
+static int batch_pick_tiles(pansim_batch* b) {
+    if (b->moveCfg >= 0) return 0;
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t cnt[6] = {0, 0, 0, 0, 0, 0};
+    if (!b->samples.empty() && hipEventSynchronize(b->samples.back().e1) != hipSuccess)
+        return batch_fail(b, PANSIM_E_DEVICE, "autotuner: event sync failed");
+    for (auto& x : b->samples) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, x.e0, x.e1) == hipSuccess) {
+            sum[x.move * 2 + x.inf] += ms;
+            cnt[x.move * 2 + x.inf]++;
+        }
+        (void)hipEventDestroy(x.e0);
+        (void)hipEventDestroy(x.e1);
+    }
+    b->samples.clear();
+    int best = 5;  // without samples: 32 movement and 8 infection agents per thread
+    double bestT = 1e300;
+    for (int k = 0; k < 6; ++k)
+        if (cnt[k] && sum[k] / cnt[k] < bestT) {
+            bestT = sum[k] / cnt[k];
+            best = k;
+        }
+    b->moveCfg = best / 2;
+    b->infCfg = best % 2;
     return 0;
 }
 
@@ -1418,27 +1553,98 @@ int pansim_batch_step(pansim_batch* b, uint32_t timestamp) {
     for (pansim_ctx* c : b->reps)
         if (timestamp - stepOfDay != c->wakeDay)
             return batch_fail(b, PANSIM_E_ARG, "pansim_batch_step: pansim_plan_locations must run at the start of every day");
-    const uint32_t n = (uint32_t)b->reps.size();
+    if (timestamp - stepOfDay != *b->hDayStart)
+        return batch_fail(b, PANSIM_E_ARG, "pansim_batch_step: the day's arguments are not this day's");
     for (pansim_ctx* c : b->reps) c->stateEpoch++;
-    BatchArgs a;
-    a.move = b->dMove;
-    a.infect = b->dInf;
-    a.moveRep = b->dTables;
-    a.moveFirst = a.moveRep + b->moveBlocks;
-    a.infRep = a.moveFirst + n;
-    a.infFirst = a.infRep + b->infBlocks;
-    a.ts = timestamp;
-    a.stepOfDay = stepOfDay;
-    a.pendPar = b->pendPar;
-    hipLaunchKernelGGL(k_move_batch, dim3(b->moveBlocks), dim3(MOVE_BLOCK), 0, b->stream, a);
-    bool anyHazard = false;
-    for (pansim_ctx* c : b->reps) anyHazard = anyHazard || c->hazardOn;
-    if (anyHazard) hipLaunchKernelGGL(k_infect_batch<true>, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
-    else hipLaunchKernelGGL(k_infect_batch<false>, dim3(b->infBlocks), dim3(INFECT_BLOCK), 0, b->stream, a);
+    if (b->moveCfg >= 0) {
+        launch_batch_step(b, stepOfDay, b->pendPar, batch_hazard(b), b->moveCfg, b->infCfg);
+    } else {
+        const int k = (int)(b->samples.size() % 6);
+        pansim_batch::TuneSample x{k / 2, k % 2, nullptr, nullptr};
+        if (hipEventCreate(&x.e0) != hipSuccess || hipEventCreate(&x.e1) != hipSuccess)
+            return batch_fail(b, PANSIM_E_DEVICE, "autotuner: hipEventCreate failed");
+        (void)hipEventRecord(x.e0, b->stream);
+        launch_batch_step(b, stepOfDay, b->pendPar, batch_hazard(b), x.move, x.inf);
+        (void)hipEventRecord(x.e1, b->stream);
+        b->samples.push_back(x);
+    }
     b->pendPar ^= 1u;
     for (pansim_ctx* c : b->reps) c->pendPar = b->pendPar;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, std::string("pansim_batch_step: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int pansim_batch_set_tiles(pansim_batch* b, uint32_t move_items, uint32_t infect_items) {
+    if (!b) return PANSIM_E_ARG;
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, "set_tiles: sync failed");
+    if (move_items == 0 && infect_items == 0) {
+        b->moveCfg = b->infCfg = -1;  // tune again
+    } else {
+        int mc = -1, ic = -1;
+        for (int k = 0; k < 3; ++k)
+            if (kMoveItems[k] == move_items) mc = k;
+        for (int k = 0; k < 2; ++k)
+            if (kInfectItems[k] == infect_items) ic = k;
+        if (mc < 0 || ic < 0) return batch_fail(b, PANSIM_E_ARG, "set_tiles: not a candidate tile size");
+        b->moveCfg = mc;
+        b->infCfg = ic;
+    }
+    for (auto& x : b->samples) {
+        (void)hipEventDestroy(x.e0);
+        (void)hipEventDestroy(x.e1);
+    }
+    b->samples.clear();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return 0;
+}
+
+int pansim_batch_tiles(pansim_batch* b, uint32_t* move_items, uint32_t* infect_items) {
+    if (!b || !move_items || !infect_items) return PANSIM_E_ARG;
+    *move_items = b->moveCfg >= 0 ? kMoveItems[b->moveCfg] : 0u;
+    *infect_items = b->infCfg >= 0 ? kInfectItems[b->infCfg] : 0u;
+    return 0;
+}
+
+int pansim_batch_run_day(pansim_batch* b, uint32_t timestamp) {
+    if (!b) return PANSIM_E_ARG;
+    if (!b->haveDay) return batch_fail(b, PANSIM_E_ARG, "pansim_batch_run_day: pansim_batch_set_day was not called");
+    const uint32_t spd = 1440u / b->dt;
+    if (timestamp % spd != 0 || timestamp != *b->hDayStart)
+        return batch_fail(b, PANSIM_E_ARG, "pansim_batch_run_day: not the first step of the day set by pansim_batch_set_day");
+    for (pansim_ctx* c : b->reps)
+        if (timestamp != c->wakeDay)
+            return batch_fail(b, PANSIM_E_ARG, "pansim_batch_run_day: pansim_plan_locations must run at the start of every day");
+    const int hazard = batch_hazard(b) ? 1 : 0;
+    if (int rc = batch_pick_tiles(b)) return rc;
+    if (!b->exec || b->graphHazard != hazard || b->graphPar != b->pendPar) {
+        // capture the day: spd (movement, infection) launch pairs and nothing
+        // else (no memset nodes: the counters the steps clear are cleared by
+        // the kernels themselves)
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return batch_fail(b, PANSIM_E_DEVICE, "pansim_batch_run_day: hipStreamBeginCapture failed");
+        for (uint32_t s = 0; s < spd; ++s) launch_batch_step(b, s, b->pendPar ^ (s & 1u), hazard != 0, b->moveCfg, b->infCfg);
+        hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+        if (e != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        b->graphHazard = hazard;
+        b->graphPar = b->pendPar;
+    }
+    hipError_t e = hipGraphLaunch(b->exec, b->stream);
+    if (e != hipSuccess) return batch_fail(b, PANSIM_E_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    b->pendPar ^= spd & 1u;
+    for (pansim_ctx* c : b->reps) {
+        c->stateEpoch += spd;
+        c->pendPar = b->pendPar;
+    }
     return 0;
 }
 

@@ -58,7 +58,37 @@ public:
     // movement + infection.
     void step() {
         const bool midnight = sims.front()->getSimTime().isMidnight();
-        if (midnight || !dayReady_) {
+        if (midnight || !dayReady_) setDay();
+        int rc = pansim_batch_step(batch, timestamp());
+        if (rc != 0) throw InitError(rc, std::string("batch step: ") + pansim_batch_error(batch));
+        for (auto& s : sims) s->finishBatchedStep();
+        if (sims.front()->getSimTime().isMidnight()) dayReady_ = false;
+    }
+
+    // One whole simulated day from its midnight: the midnight blocks, then
+    // the day's steps as one replay of the captured day graph.
+    void day() {
+        if (!sims.front()->getSimTime().isMidnight())
+            throw InitError(ERR_BAD_ARG, "a whole-day batch starts at a midnight");
+        setDay();
+        int rc = pansim_batch_run_day(batch, timestamp());
+        if (rc != 0) throw InitError(rc, std::string("batch day: ") + pansim_batch_error(batch));
+        const unsigned spd = sims.front()->getSimTime().getStepsPerDay();
+        for (auto& s : sims)
+            for (unsigned k = 0; k < spd; ++k) s->finishBatchedStep();
+        dayReady_ = false;
+    }
+
+    unsigned totalSteps() const {
+        return sims.front()->lengthOfSimulationWeeks * 7u * sims.front()->getSimTime().getStepsPerDay();
+    }
+
+private:
+    bool dayReady_ = false;
+
+    // midnight blocks where due, then this day's batch arguments
+    void setDay() {
+        {
             for (auto& s : sims) s->midnightIfDue();
             std::vector<pansim_step_params> p;
             std::vector<uint32_t> mask;
@@ -76,18 +106,7 @@ public:
             if (rc != 0) throw InitError(rc, std::string("batch day parameters: ") + pansim_batch_error(batch));
             dayReady_ = true;
         }
-        int rc = pansim_batch_step(batch, timestamp());
-        if (rc != 0) throw InitError(rc, std::string("batch step: ") + pansim_batch_error(batch));
-        for (auto& s : sims) s->finishBatchedStep();
-        if (sims.front()->getSimTime().isMidnight()) dayReady_ = false;
     }
-
-    unsigned totalSteps() const {
-        return sims.front()->lengthOfSimulationWeeks * 7u * sims.front()->getSimTime().getStepsPerDay();
-    }
-
-private:
-    bool dayReady_ = false;
 };
 
 }  // namespace ps

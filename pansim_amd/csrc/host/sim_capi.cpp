@@ -160,6 +160,15 @@ int pansim_ensemble_steps(pansim_ensemble* e, uint32_t n) {
     });
 }
 
+pansim_batch* pansim_ensemble_batch(pansim_ensemble* e) { return e ? e->ens->batch : nullptr; }
+
+int pansim_ensemble_days(pansim_ensemble* e, uint32_t n) {
+    if (!e) return PANSIM_E_ARG;
+    return guarded(e, [&] {
+        for (uint32_t i = 0; i < n; ++i) e->ens->day();
+    });
+}
+
 int pansim_ensemble_run(pansim_ensemble* e) {
     if (!e) return PANSIM_E_ARG;
     return guarded(e, [&] {

@@ -126,3 +126,38 @@ def test_autograd_function_matches_oracle():
     assert abs(J.item() - Jo) <= 1e-5 * Jo
     assert abs(k.grad.item() - dko) <= 1e-5 * abs(dko)
     assert np.all(np.abs(m.grad.numpy() - dmo[:45]) <= 1e-5 * np.maximum(np.abs(dmo[:45]), 1e-12))
+
+
+@pytest.mark.gpu
+def test_device_gradient_against_device_central_differences():
+    """The reduced config-5 shape on the device: 32 batched replicas of
+    20,000 agents (seed_i = i), 3 weeks, hazard accumulation with the
+    finite-difference probes on: for every replica the device's dJ/dk and
+    dJ/dm_work equal the central differences (J(theta(1+h)) - J(theta(1-h))) /
+    (2 h theta) of J along the same trajectory (h = 1e-3, common random
+    numbers: one run carries both) within 1e-3 relative."""
+    h, work = 1e-3, 4
+    reps = [pansim_amd.synthetic_args(20_000, 3, i, i + 1, extra=["-k", "0.002", "--synthInfected", "0.01"])
+            for i in range(32)]
+    ens = pansim_amd.Ensemble(reps)
+    try:
+        views = [ens.replica(r) for r in range(32)]
+        for v in views:
+            v.set_location_type_multipliers(np.ones(256, dtype=np.float32))
+            v.hazard_enable(True)
+            v.hazard_set_probe(h, work)
+        ens.steps(3 * 7 * 144)
+        k = np.float64(np.float32(0.002))
+        worst = 0.0
+        for v in views:
+            out = v.hazard_with_probes()
+            J, dk, dm = out[0], out[1], out[2:258]
+            fd_k = (out[258] - out[259]) / (2 * h * k)
+            fd_m = (out[260] - out[261]) / (2 * h * 1.0)
+            assert J > 10.0
+            assert abs(dk - fd_k) <= 1e-3 * abs(fd_k), (dk, fd_k)
+            assert abs(dm[work] - fd_m) <= 1e-3 * abs(fd_m), (dm[work], fd_m)
+            worst = max(worst, abs(dk - fd_k) / abs(fd_k), abs(dm[work] - fd_m) / abs(fd_m))
+        print(f"largest relative difference gradient vs central differences: {worst:.2e}")
+    finally:
+        ens.close()
