@@ -338,6 +338,16 @@ int pansim_device_expf(const float* in, float* out, size_t n);
 int pansim_device_log(const double* in, double* out, size_t n);
 int pansim_device_philox(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4, size_t n);
 
+/* pansim_device_philox without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); it has no counterpart in the reference.  Counters and
+ * blocks are staged in a fixed device array of 2 x 16 B x
+ * PANSIM_PHILOX_NOALLOC_MAX (2 MiB) that each GPU keeps until the process
+ * ends; more counters return PANSIM_E_ARG (message in pansim_last_error(NULL)).
+ * Calls on one GPU share that array: calls that overlap on one GPU race. */
+#define PANSIM_PHILOX_NOALLOC_MAX 65536u
+int pansim_device_philox_noalloc(const uint32_t* ctr4, uint32_t k0, uint32_t k1, uint32_t* out4, size_t n);
+
 /* ---- Hazard gradient -----------------------------------------------------
  * Calibration support, with no counterpart in the reference: per location
  * type multipliers m_t of the location infectiousness (type id < 256; all 1

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1257,6 +1258,49 @@ int pansim_device_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t
     hipFree(din);
     hipFree(dout);
     return e == hipSuccess ? 0 : PANSIM_E_DEVICE;
+}
+
+// Philox blocks without any allocation: the counters and the blocks are
+// staged in `scratch`, a file-scope device array (each GPU holds its own copy
+// until the process ends).  Its address on the current device comes from
+// hipGetSymbolAddress at the first call on that device and is kept per
+// device; host code uses only that address, never the array's name.  Calls
+// that overlap on one GPU share the array and race.
+static __device__ __align__(256) uint32_t scratch[2 * 4 * PANSIM_PHILOX_NOALLOC_MAX];
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[64];
+
+int pansim_device_philox_noalloc(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, size_t n) {
+    if (n == 0) return 0;
+    if (!ctr || !out) return PANSIM_E_ARG;
+    if (n > PANSIM_PHILOX_NOALLOC_MAX) {
+        g_create_error = "pansim_device_philox_noalloc: " + std::to_string(n) + " counters exceed the scratch of " +
+                         std::to_string(PANSIM_PHILOX_NOALLOC_MAX);
+        return PANSIM_E_ARG;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return PANSIM_E_DEVICE;
+    uint32_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                g_create_error = std::string("pansim_device_philox_noalloc: hipGetSymbolAddress failed: ") +
+                                 hipGetErrorString(e);
+                return PANSIM_E_DEVICE;
+            }
+            g_scratch_ptr[dev] = (uint32_t*)p;
+        }
+        base = g_scratch_ptr[dev];
+    }
+    uint32_t* din = base;
+    uint32_t* dout = base + 4 * (size_t)PANSIM_PHILOX_NOALLOC_MAX;
+    if (hipMemcpy(din, ctr, n * 16, hipMemcpyHostToDevice) != hipSuccess) return PANSIM_E_DEVICE;
+    hipLaunchKernelGGL(k_philox, dim3(blocks_for(n)), dim3(256), 0, 0, din, k0, k1, dout, n);
+    if (hipGetLastError() != hipSuccess) return PANSIM_E_DEVICE;
+    return hipMemcpy(out, dout, n * 16, hipMemcpyDeviceToHost) == hipSuccess ? 0 : PANSIM_E_DEVICE;
 }
 
 // ------------------------------------------------------------ hazard gradient
