@@ -494,9 +494,6 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     TRY(dalloc(c, &c->S.pendList, c->S.pendCap));
     TRY(dalloc(c, &c->S.pendCount, 2));
     HIP_OR_FAIL(c, hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), c->stream));
-    TRY(dalloc(c, &c->S.rareList, (size_t)d->n_agents + 16u));
-    TRY(dalloc(c, &c->S.rareCount, 2));
-    HIP_OR_FAIL(c, hipMemsetAsync(c->S.rareCount, 0, 2 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &c->counters, S_COUNT + 16));
     if (hipHostMalloc((void**)&c->hostCounters, (S_COUNT + 16) * sizeof(unsigned long long), hipHostMallocDefault) !=
         hipSuccess) {
@@ -664,7 +661,6 @@ int pansim_movement_step(pansim_ctx* c, const pansim_step_params* p) {
         c->pipeOpen = true;
     }
     hipLaunchKernelGGL(k_move_tile, dim3(blocks_for(c->A.n, MOVE_TILE)), dim3(MOVE_BLOCK), 0, c->stream, m);
-    hipLaunchKernelGGL(k_move_rare, dim3(MOVE_RARE_BLOCKS), dim3(MOVE_BLOCK), 0, c->stream, m);
     c->resolvePending = true;
     c->resolvePar = m.pendPar;
     if (perLaunch) {
@@ -1454,8 +1450,7 @@ int pansim_batch_create(pansim_batch** out, pansim_ctx* const* replicas, uint32_
         b->ownStreams.push_back(c->stream);
         c->stream = b->stream;
         // at rest neither pending-write counter holds anything: restart all at parity 0
-        if (hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess ||
-            hipMemsetAsync(c->S.rareCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess)
+        if (hipMemsetAsync(c->S.pendCount, 0, 2 * sizeof(uint32_t), b->stream) != hipSuccess)
             return bail(PANSIM_E_DEVICE, "pansim_batch_create: memset failed");
         c->pendPar = 0;
     }
@@ -1526,7 +1521,6 @@ extern "C++" {
 template <uint32_t MI, uint32_t II>
 static void launch_batch_sized(pansim_batch* b, const BatchArgs& a, uint32_t mc, uint32_t ic, bool hazard) {
     hipLaunchKernelGGL(k_move_batch<MI>, dim3(b->moveBlocks[mc]), dim3(MOVE_BLOCK), 0, b->stream, a);
-    hipLaunchKernelGGL(k_move_rare_batch, dim3(a.nRep * MOVE_RARE_BATCH_BLOCKS), dim3(MOVE_BLOCK), 0, b->stream, a);
     if (hazard) hipLaunchKernelGGL((k_infect_batch<true, II>), dim3(b->infBlocks[ic]), dim3(INFECT_BLOCK), 0, b->stream, a);
     else hipLaunchKernelGGL((k_infect_batch<false, II>), dim3(b->infBlocks[ic]), dim3(INFECT_BLOCK), 0, b->stream, a);
 }
@@ -1545,6 +1539,7 @@ static void launch_batch_step(pansim_batch* b, uint32_t stepOfDay, uint32_t pend
     a.stepOfDay = stepOfDay;
     a.pendPar = pendPar;
     a.nRep = n;
+    a.moveBlocks = b->moveBlocks[mc];
     a.infBlocks = b->infBlocks[ic];
     switch (mc * 2 + ic) {
     case 0: launch_batch_sized<kMoveItems[0], kInfectItems[0]>(b, a, 0, 0, hazard); break;

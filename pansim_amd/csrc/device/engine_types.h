@@ -181,8 +181,6 @@ struct StepScratch {
     uint32_t* pendList;     // locations with a pending quarantine write
     uint32_t* pendCount;    // [2]: entries of pendList, by step parity
     uint32_t pendCap;
-    uint32_t* rareList;     // movers of this step left to the full movement routine (up to every agent)
-    uint32_t* rareCount;    // [2]: entries of rareList, by step parity
 };
 
 }  // namespace dev

@@ -287,15 +287,12 @@ __device__ __forceinline__ uint32_t div_dt(const MoveArgs& m, uint32_t x) {
 }
 
 constexpr uint32_t MOVE_BLOCK = 256;
-// 8192 agents per block: 1221 blocks for 10M agents, one round of the
-// 256 CUs x 5 resident blocks, so that a step with few movers (latency
-// bound: one mover's dependent loads) costs one round, not two
+// 32 chunks of 256 agents per block: 1221 blocks for 10M agents, one round
+// of the 256 CUs x 5 resident blocks, so that a step with few movers
+// (latency bound: one mover's dependent loads) costs one round, not two
 constexpr uint32_t MOVE_ITEMS = 32;
 constexpr uint32_t MOVE_TILE = MOVE_BLOCK * MOVE_ITEMS;  // agents per block
-constexpr uint32_t MOVE_CHUNKS = MOVE_ITEMS / 8;         // 16-byte wake vectors per thread
 constexpr uint32_t MOVE_CTR_SLOTS = 64;
-static_assert(MOVE_ITEMS % 8 == 0, "the wake scan loads vectors of 8 wake steps");
-static_assert(MOVE_TILE <= 65536, "movers are listed as 16-bit offsets in the tile");
 
 __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, uint32_t loc, uint32_t until) {
     unsigned long long key = ((unsigned long long)agent << 32) | (unsigned long long)until;
@@ -643,228 +640,68 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     return true;
 }
 
-// do_movement for the common mover: an agent not under curfew handling, not
-// in a serious, dead or hospital state, not quarantined, diagnosed or kept
-// home for child care, not recently diagnosed, at and going to locations
-// that are not closed or quarantined today, under no lockdown of the
-// non-vaccinated, and not diagnosed on arrival.  It follows the same steps
-// with the same draws as do_movement; whenever the agent turns out not to be
-// common it returns -1 before it has written anything, and the caller runs
-// the full routine, which draws the same numbers from a fresh stream.
-// Returns 1 when the agent changed location, 0 when not, -1 to delegate.
-__device__ __forceinline__ int move_common(const MoveArgs& m, uint32_t i) {
-    const AgentRec r = m.A.rec[i];
-    const Agents& A = m.A;
-    const Locs& L = m.L;
-    const Tables& T = m.T;
-    const pansim_location_types& lt = T.lt;
-    const uint32_t ts = m.p.timestamp;
-    const uint32_t w = r.wb;
-    if (m.curfewStep || m.p.enable_curfew || m.p.lockdown_nonvacc_active || w >= 3u) return -1;
-    if (r.hospTs <= ts && r.hospUntil > ts) return -1;
-    if (r.flags & (F_QUARANTINED | F_DIAGNOSED | F_NO_WORK)) return -1;
-    const uint32_t qp = m.p.quarantine_policy;
-    if (qp > 0 && r.diagTs > 0 && (ts - r.diagTs) < m.qLen) return -1;
-    const uint32_t oldLoc = A.location[i];
-    if (qp > 1 && loc_special(L, oldLoc)) return -1;
-    const CsrReg csr = load_csr(A, i, r);
-    const uint32_t minutes = m.minutes;
-    const TD simTime = td_from_min(minutes);
-    const uint32_t stepsUntilMidnight = m.stepsUntilMidnight;
-    Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
-    uint32_t loc = oldLoc, sum = 0, newType = NOLOC;
-    const uint32_t off = 7u * w + m.p.day_of_week + (uint32_t)r.type * 28u;
-    const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff + off];
-    const int ab = win.x, ae = win.y;
-    if (ab == -1 && ae == -1) {  // Case 1 (home: the agent is not in a serious state)
-        newType = lt.home;
-        loc = csr.home;
-        sum = stepsUntilMidnight;
-    }
-    if (ab != -1) {  // Cases 2 and 4
-        const uint32_t nPot = (uint32_t)((ae == -1 ? ab + 1 : ae) - ab);
-        uint32_t idx = 0;
-        if (nPot > 1) {
-            const double u = rs.real(1.0);
-            double thr = T.ev[ab].chance;
-            while (u > thr && idx < nPot) {
-                idx++;
-                if (idx < nPot) thr += T.ev[ab + idx].chance;
-            }
-            if (idx == nPot) idx--;
-        }
-        const pansim_event& ev = T.ev[ab + idx];
-        newType = ev.loc_type;
-        const TD dur = ev_dur(ev);
-        const uint32_t newLoc = find_loc(A, L, lt, csr, newType, nPot == 1, &rs);
-        if ((newType == lt.school || newType == lt.classroom) && r.age >= m.p.school_age_restriction) return -1;
-        if (newType != lt.work && loc_special(L, newLoc)) return -1;  // perhaps closed
-        if (newType == lt.school) {
-            const uint32_t cls = find_loc(A, L, lt, csr, lt.classroom, 0, &rs);
-            if (cls != NOLOC && loc_special(L, cls)) return -1;  // perhaps closed
-        }
-        loc = newLoc;
-        if (dur.h > 24) {
-            sum = stepsUntilMidnight;
-        } else if (ae == -1) {
-            const TD arrive = td_plus(simTime, dur);
-            sum = arrive.h >= 24 ? stepsUntilMidnight : div_dt(m, td_min(dur));
-        } else {
-            pansim_event ne;
-            if ((uint32_t)ae < m.nEvents) ne = T.ev[ae];
-            else {
-                ne.loc_type = 0;
-                ne.chance = -1.0f;
-                ne.start_h = ne.start_m = ne.end_h = ne.end_m = ne.dur_h = ne.dur_m = 0;
-            }
-            const TD arrive = td_plus(simTime, dur);
-            if (td_lt(arrive, ev_start(ne))) sum = div_dt(m, td_min(dur));
-            else if (td_lt(ev_end(ne), arrive)) sum = div_dt(m, td_min(td_minus(ev_end(ne), simTime))) - 1u;
-            else {
-                const uint32_t stp = div_dt(m, td_min(td_minus(ev_end(ne), arrive)));
-                sum = div_dt(m, td_min(dur)) + rs.uint_below(stp);
-            }
-        }
-    }
-    if (ab == -1 && ae != -1) {  // Case 3
-        const pansim_event& ne = T.ev[ae];
-        const uint32_t lenSteps = div_dt(m, td_min(td_minus(ev_end(ne), ev_start(ne))));
-        const uint32_t delay = rs.uint_below(lenSteps);
-        sum = div_dt(m, td_min(td_minus(ev_start(ne), simTime))) + delay;
-        if (sum < m.halfHourSteps) {
-        } else if (sum < m.hourSteps) {
-            newType = lt.public_space;
-            loc = find_loc(A, L, lt, csr, lt.public_space, 0, &rs);
-        } else {
-            newType = lt.home;
-            loc = csr.home;
-        }
-    }
-    if (qp > 1 && loc_special(L, loc)) return -1;  // perhaps arrived at a quarantined location
-    const float xInf = T.stInf[r.state] * T.varMul[r.variant];
-    // diagnosed on arrival: only an infectious agent (the doctor's draw is the
-    // routine's last draw, so leaving it out for the others changes nothing)
-    if ((newType == lt.hospital || newType == lt.doctor) && xInf > 0.0f) return -1;
-    // --- no delegation from here on: the writes
-    if (csr.home != NOLOC && loc != csr.home && sum > m.hourSteps && (r.flags & F_STAYED_HOME))
-        A.rec[i].flags = (uint8_t)(r.flags & ~F_STAYED_HOME);
-    const uint32_t dayStart = m.dayStart;
-    const uint32_t wake = wake_dec(ts, sum);
-    const uint16_t wake16 = (wake == NEVER || wake - dayStart >= m.spd) ? WAKE_NONE : (uint16_t)(wake - dayStart);
-    if (wake16 != (uint16_t)m.stepOfDay) A.wake[i] = wake16;
-    if (loc == NOLOC) loc = oldLoc;
-    if (loc == oldLoc) return 0;
-    A.location[i] = loc;
-    if (xInf != 0.0f) {
-        const long long q = fixed32(xInf);
-        atomicAdd((unsigned long long*)loc_sum(L, r.variant, oldLoc), (unsigned long long)(-q));
-        atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)q);
-        mark_visited(L, loc);
-    }
-    atomicSub(&L.dyn[oldLoc].count, 1u);
-    atomicAdd(&L.dyn[loc].count, 1u);
-    return 1;
-}
-
-// One movement step over a tile of MOVE_TILE consecutive agents per block:
-// the agents whose countdown reaches zero at this step (everyone on the
-// curfew step) are compacted into LDS, then processed by the same block with
-// move_common; the movers it leaves to the full routine are appended to
-// S.rareList and processed by k_move_rare (the full routine's registers would
-// otherwise limit the waves in flight for every mover).
-// the agents whose countdown reaches zero at this step (everyone on the
-// curfew step) are compacted into LDS, then processed by the same block.
-// The order in which movers are processed does not change any result: the
-// per-agent writes are private, the per-location sums are integer atomics and
-// the location quarantine writes are deferred to k_resolve_pending.
-// Evaluated / relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
+// One movement step for block blk of the nBlk blocks working on this
+// population.  The block takes the 256-agent chunks blk, blk + nBlk,
+// blk + 2 nBlk, ... (ITEMS of them): agents that move at the same steps (one
+// schedule's agents, which a population may store next to each other) are
+// spread over every block instead of loading a few.  The agents whose wake
+// step is this step (everyone on the curfew step) are compacted into LDS,
+// then processed by the same block with do_movement.  The order in which
+// movers are processed does not change any result: the per-agent writes are
+// private, the per-location sums are integer atomics and the location
+// quarantine writes are deferred to the end of the step.  Evaluated /
+// relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
 template <uint32_t ITEMS>
-__device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t tile) {
-    constexpr uint32_t MOVE_TILE = MOVE_BLOCK * ITEMS;  // agents of this block
-    constexpr uint32_t MOVE_CHUNKS = ITEMS / 8;         // 16-byte wake vectors per thread
-    static_assert(ITEMS % 8 == 0 && MOVE_TILE <= 65536, "wake vectors of 8; 16-bit offsets in the tile");
-    __shared__ uint16_t list[MOVE_TILE];  // movers, as offsets in the tile
+__device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t blk, uint32_t nBlk) {
+    static_assert(MOVE_BLOCK == 256 && ITEMS <= 256, "list entries are (chunk << 8) | thread");
+    __shared__ uint16_t list[MOVE_BLOCK * ITEMS];  // movers, as (chunk of this block << 8) | thread
     __shared__ uint32_t nList, nMoved;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
-    const uint32_t base = tile * MOVE_TILE;
     const uint32_t n = m.A.n;
     if (tid == 0) {
         nList = 0;
         nMoved = 0;
     }
-    // MOVE_CHUNKS 16-byte loads per thread: agents base + k * (MOVE_TILE / MOVE_CHUNKS) + tid * 8 + j
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    const v4u* wv = reinterpret_cast<const v4u*>(m.A.wake + base);
-    v4u w[MOVE_CHUNKS];
+    // one 2-byte wake load per chunk and thread; a wave's load covers 64
+    // consecutive agents
+    uint16_t w[ITEMS];
 #pragma unroll
-    for (uint32_t k = 0; k < MOVE_CHUNKS; ++k) w[k] = __builtin_nontemporal_load(&wv[k * MOVE_BLOCK + tid]);
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+        const uint32_t i = (blk + k * nBlk) * MOVE_BLOCK + tid;
+        w[k] = i < n ? __builtin_nontemporal_load(&m.A.wake[i]) : (uint16_t)WAKE_NONE;
+    }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (uint32_t k = 0; k < MOVE_CHUNKS; ++k) {
-        const uint32_t words[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) {
-            const uint32_t i = base + k * (MOVE_TILE / MOVE_CHUNKS) + tid * 8u + j;
-            const uint32_t w16 = (words[j >> 1] >> (16u * (j & 1u))) & 0xFFFFu;
-            bool mover = i < n && (m.curfewStep || w16 == m.stepOfDay);
-            unsigned long long mask = __ballot(mover);
-            if (mask != 0ull) {
-                uint32_t pos = 0;
-                if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
-                pos = __shfl(pos, 0, 64);
-                if (mover) list[pos + (uint32_t)__popcll(mask & below)] = (uint16_t)(i - base);
-            }
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+        const uint32_t i = (blk + k * nBlk) * MOVE_BLOCK + tid;
+        const bool mover = i < n && (m.curfewStep || w[k] == (uint16_t)m.stepOfDay);
+        const unsigned long long mask = __ballot(mover);
+        if (mask != 0ull) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
+            pos = __shfl(pos, 0, 64);
+            if (mover) list[pos + (uint32_t)__popcll(mask & below)] = (uint16_t)((k << 8) | tid);
         }
     }
     __syncthreads();
     const uint32_t cnt = nList;
     uint32_t moved = 0;
-    if (tile == 0 && tid == 0) m.S.rareCount[m.pendPar ^ 1u] = 0u;  // the previous step's list, consumed
-    for (uint32_t j0 = 0; j0 < cnt; j0 += MOVE_BLOCK) {
-        const uint32_t j = j0 + tid;
-        const uint32_t i = j < cnt ? base + list[j] : 0u;
-        const int res = j < cnt ? move_common(m, i) : 0;
-        if (res > 0) ++moved;
-        // one append per wave for the movers left to the full routine
-        const unsigned long long rare = __ballot(res < 0);
-        if (rare != 0ull) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&m.S.rareCount[m.pendPar], (uint32_t)__popcll(rare));
-            pos = __shfl(pos, 0, 64);
-            if (res < 0) m.S.rareList[pos + (uint32_t)__popcll(rare & below)] = i;
-        }
+    for (uint32_t j = tid; j < cnt; j += MOVE_BLOCK) {
+        const uint32_t e = list[j];
+        moved += do_movement(m, (blk + (e >> 8) * nBlk) * MOVE_BLOCK + (e & 0xFFu)) ? 1u : 0u;
     }
     if (moved) atomicAdd(&nMoved, moved);
     __syncthreads();
     if (tid == 0 && cnt) {
-        unsigned long long* c = m.counters + 2u * (tile % MOVE_CTR_SLOTS);
+        unsigned long long* c = m.counters + 2u * (blk % MOVE_CTR_SLOTS);
         atomicAdd(&c[0], (unsigned long long)cnt);
         if (nMoved) atomicAdd(&c[1], (unsigned long long)nMoved);
     }
 }
 
-__global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) { move_tile<MOVE_ITEMS>(m, blockIdx.x); }
-
-// The movers of this step that move_common left to the full routine, block
-// `blk` of `nBlk` blocks working on this population; their relocations are
-// added to the step's counters.
-__device__ __forceinline__ void move_rare(const MoveArgs& m, uint32_t blk, uint32_t nBlk) {
-    __shared__ uint32_t nMoved;
-    if (threadIdx.x == 0) nMoved = 0;
-    __syncthreads();
-    const uint32_t cnt = m.S.rareCount[m.pendPar];
-    uint32_t moved = 0;
-    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < cnt; j += nBlk * blockDim.x)
-        moved += do_movement(m, m.S.rareList[j]) ? 1u : 0u;
-    if (moved) atomicAdd(&nMoved, moved);
-    __syncthreads();
-    if (threadIdx.x == 0 && nMoved) atomicAdd(&m.counters[2u * (blk % MOVE_CTR_SLOTS) + 1u], (unsigned long long)nMoved);
-}
-
-constexpr uint32_t MOVE_RARE_BLOCKS = 64;  // blocks of a population's rare-mover launch
-__global__ void __launch_bounds__(MOVE_BLOCK) k_move_rare(MoveArgs m) { move_rare(m, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(MOVE_BLOCK) k_move_tile(MoveArgs m) { move_tile<MOVE_ITEMS>(m, blockIdx.x, gridDim.x); }
 
 // End of the step's movement: the deferred quarantine writes; the highest
 // writing agent index of each location wins.  The step's pending list count
@@ -1132,8 +969,9 @@ struct BatchArgs {
     const uint32_t* dayStart;   // device word: timestamp of the day's first step (set with the day's arguments)
     uint32_t stepOfDay;
     uint32_t pendPar;
-    uint32_t nRep;       // replicas
-    uint32_t infBlocks;  // infection blocks of all replicas
+    uint32_t nRep;        // replicas
+    uint32_t moveBlocks;  // movement blocks of all replicas
+    uint32_t infBlocks;   // infection blocks of all replicas
 };
 
 template <uint32_t ITEMS>
@@ -1142,17 +980,9 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_batch(BatchArgs b) {
     MoveArgs m = b.move[r];
     move_set_step(m, *b.dayStart + b.stepOfDay, b.stepOfDay);
     m.pendPar = b.pendPar;
-    move_tile<ITEMS>(m, blockIdx.x - b.moveFirst[r]);
-}
-
-// The rare movers of every replica: MOVE_RARE_BATCH_BLOCKS blocks per replica.
-constexpr uint32_t MOVE_RARE_BATCH_BLOCKS = 8;
-__global__ void __launch_bounds__(MOVE_BLOCK) k_move_rare_batch(BatchArgs b) {
-    const uint32_t r = blockIdx.x / MOVE_RARE_BATCH_BLOCKS;
-    MoveArgs m = b.move[r];
-    move_set_step(m, *b.dayStart + b.stepOfDay, b.stepOfDay);
-    m.pendPar = b.pendPar;
-    move_rare(m, blockIdx.x % MOVE_RARE_BATCH_BLOCKS, MOVE_RARE_BATCH_BLOCKS);
+    // this replica's blocks at this tile size
+    const uint32_t nBlk = (r + 1 < b.nRep ? b.moveFirst[r + 1] : b.moveBlocks) - b.moveFirst[r];
+    move_tile<ITEMS>(m, blockIdx.x - b.moveFirst[r], nBlk);
 }
 
 template <bool HAZARD, uint32_t ITEMS>
