@@ -54,7 +54,7 @@ __global__ void k_init_agents(Agents A, Tables T, uint64_t seed, uint32_t replic
     A.rec[i].variant = 0;
     update_meta(A, T, i, st);
     for (uint32_t v = 0; v < T.nVar; ++v) A.slot6[(size_t)v * A.n + i] = A.scal[i];
-    Stream rs(seed, i, 0, PH_INIT_DAYS, replica);
+    Stream rs(seed, A.id[i], 0, PH_INIT_DAYS, replica);
     A.days[i] = (int16_t)calc_days(T, A.prog[i], st, 0, rs);
     bool inf = T.stInfected[st] != 0;
     A.infTs[i] = inf ? 0u : NEVER;
@@ -205,7 +205,7 @@ __global__ void k_flag_locations(TestArgs a) {
     uint32_t school = find_loc(a.A, a.L, lt, csr, lt.school, 0, nullptr);
     if (school != NOLOC && ok(school)) {
         a.L.rec[school].flag = 1;
-        Stream rs(a.seed, i, a.ts, PH_TEST_FLAG, a.replica);
+        Stream rs(a.seed, a.A.id[i], a.ts, PH_TEST_FLAG, a.replica);
         uint32_t cls = find_loc(a.A, a.L, lt, csr, lt.classroom, 0, &rs);
         if (cls != NOLOC && ok(cls)) a.L.rec[cls].flag = 1;
     }
@@ -232,7 +232,7 @@ __global__ void k_do_testing(TestArgs a) {
         const bool skip = ts > a.delay * day && lastTest != NEVER && lastTest > ts - a.delay * day;
         const uint8_t st = r.state;
         if (!skip && !(r.wb == 4 || (fl & F_DIAGNOSED))) {  // wb: T.stWB[st], kept in the record
-            Stream rs(a.seed, i, ts, PH_TESTING, a.replica);
+            Stream rs(a.seed, A.id[i], ts, PH_TESTING, a.replica);
             const CsrReg csr = load_csr(A, i, r);
             const uint32_t home = csr.home;
             const uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
@@ -307,7 +307,7 @@ __global__ void k_progression(Agents A, Tables T, uint64_t seed, uint32_t replic
     if (st == 7) progS = T.deathScaling[v] / scaling_symptoms(A, i, v, st);
     else progS = T.progScaling[v];
     float scaling = scaling_symptoms(A, i, v, st) * progS;
-    Stream rs(seed, i, ts, PH_PROGRESSION, replica);
+    Stream rs(seed, A.id[i], ts, PH_PROGRESSION, replica);
     if (days == -2) {
         days = (int16_t)calc_days(T, prog, st, v, rs);
         if (v >= 3) {
@@ -526,7 +526,7 @@ __global__ void k_vacc_categories(VaccCategoryArgs a) {
     for (uint32_t p = 0; p < a.nPasses; ++p) {
         const uint32_t g = a.passes[p];
         if (!in_vacc_category(a.A, a.L, i, r, g, pre)) continue;
-        Stream rs(a.seed, i, 0, PH_IMM_CATEGORY, a.replica);
+        Stream rs(a.seed, a.A.id[i], 0, PH_IMM_CATEGORY, a.replica);
         round = rs.unit() < (double)a.level[g] ? a.round[g] : (uint8_t)255;
         break;
     }
@@ -597,7 +597,7 @@ __global__ void k_booster_apply(VaccDayArgs a) {
     int g = age_group_of(r.age);
     if (g < 0 || !booster_eligible(a, i, r)) return;
     const float prob = a.prob[g];
-    Stream rs(a.seed, i, a.tsKey, PH_VACCINATION, a.replica);
+    Stream rs(a.seed, a.A.id[i], a.tsKey, PH_VACCINATION, a.replica);
     rs.draw = a.draw;
     if (prob == 1.0f || rs.unit() < (double)prob) {
         a.A.immTs[i] = a.ts;
@@ -630,7 +630,7 @@ __global__ void k_vacc_apply(VaccDayArgs a) {
     if (a.A.immRound[i] != a.round) return;
     const AgentRec r = a.A.rec[i];
     if (!vacc_eligible(a, i, r)) return;
-    Stream rs(a.seed, i, a.tsKey, PH_VACCINATION, a.replica);
+    Stream rs(a.seed, a.A.id[i], a.tsKey, PH_VACCINATION, a.replica);
     rs.draw = a.draw;
     if (a.prob[0] == 1.0f || rs.unit() < (double)a.prob[0]) {
         a.A.immTs[i] = a.ts;
@@ -666,7 +666,7 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
     if (i >= A.n) return;
     uint8_t st = A.rec[i].state;
     if (T.stWB[st] == 4) return;
-    Stream rs(seed, i, ts, PH_OTHER_DISEASE, replica);
+    Stream rs(seed, A.id[i], ts, PH_OTHER_DISEASE, replica);
     uint8_t age = A.rec[i].age;
     uint32_t ag = age < 5 ? 0 : age < 15 ? 1 : age < 30 ? 2 : age < 60 ? 3 : age < 70 ? 4 : age < 80 ? 5 : 6;
     uint32_t notSex = A.sex[i] ? 0u : 1u;
@@ -847,7 +847,7 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
     uint8_t fl = (uint8_t)((r.flags | F_STAYED_HOME) & ~F_NO_WORK);
     if (schoolwork && !(fl & F_QUARANTINED) && !(r.diagTs > 0 && !a.T.stInfected[r.state])) {
         const pansim_location_types& lt = a.T.lt;
-        Stream rs(a.seed, i, a.ts, PH_PLAN_SCHOOLWORK, a.replica);
+        Stream rs(a.seed, a.A.id[i], a.ts, PH_PLAN_SCHOOLWORK, a.replica);
         const CsrReg csr = load_csr(A, i, r);
         // the planning lookup (home, school, classroom first-match): a
         // workplace is drawn when the agent has several, in this order
@@ -939,8 +939,9 @@ __global__ void k_plan_nowork_claim(PlanArgs a, const uint32_t* adults, uint32_t
     if (j >= n) return;
     const uint32_t i = adults[j];
     uint32_t h = A.rec[i].home;
-    // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent + 1)
-    if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, i + 1u);
+    // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent's
+    // input index + 1)
+    if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, A.id[i] + 1u);
 }
 
 __global__ void k_plan_nowork_resolve(PlanArgs a, const uint32_t* adults, uint32_t n) {
@@ -949,7 +950,7 @@ __global__ void k_plan_nowork_resolve(PlanArgs a, const uint32_t* adults, uint32
     if (j >= n) return;
     const uint32_t i = adults[j];
     uint32_t h = A.rec[i].home;
-    if (h != NOLOC && a.L.rec[h].noWork == i + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
+    if (h != NOLOC && a.L.rec[h].noWork == A.id[i] + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
 }
 
 // ------------------------------------------------------------ closures
@@ -991,7 +992,7 @@ __global__ void k_expose_mutation(Agents A, Tables T, uint64_t seed, uint32_t re
     if (i >= A.n) return;
     float s = A.susc[(size_t)v * A.suscStride + i];
     if (!(s > 0.0f)) return;
-    Stream rs(seed, i, ts, PH_MUTATION, replica);
+    Stream rs(seed, A.id[i], ts, PH_MUTATION, replica);
     if (rs.unit() < fraction * (double)s) {
         uint8_t fi = (uint8_t)T.firstInfected;
         A.rec[i].state = fi;

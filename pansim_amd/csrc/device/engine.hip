@@ -3,6 +3,7 @@
 // of every per-step and per-midnight routine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,7 @@ struct pansim_ctx {
     float* dumpInfVal = nullptr;
     uint32_t dt = 10;
     uint32_t wakeDay = 0;  // first step of the day the stored wake steps count from
+    std::vector<uint32_t> perm;  // device agent index -> input agent index
 };
 
 // A batch of replicas stepped together (include/pansim_hip.h): the member
@@ -362,6 +364,53 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
             c->err = "pansim_create: possible location out of range";
             return bail(PANSIM_E_ARG);
         }
+    // ---- storage order: agents grouped by schedule (a counting sort by agent
+    // type, input order within a type), so that the agents that move at one
+    // step sit next to each other; every agent array below is uploaded in
+    // this order and c->perm (device index -> input index) maps back
+    std::vector<uint8_t> pAge(N), pSex(N), pPre(N), pType(N), pProg(N), pState(N), pDiag(N), pHcw(N), pPType(d->n_csr);
+    std::vector<uint32_t> pLoc(N), pOff((size_t)N + 1), pPLoc(d->n_csr);
+    std::vector<float> pScal(N);
+    pansim_agents ap = *a;
+    {
+        uint32_t first[257] = {0};
+        for (uint32_t k = 0; k < N; ++k) first[a->agent_type[k] + 1]++;
+        for (int b = 0; b < 256; ++b) first[b + 1] += first[b];
+        c->perm.resize(N);
+        for (uint32_t k = 0; k < N; ++k) c->perm[first[a->agent_type[k]]++] = k;
+        pOff[0] = 0;
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint32_t k = c->perm[i];
+            pAge[i] = a->age[k];
+            pSex[i] = a->sex[k];
+            pPre[i] = a->precond[k];
+            pType[i] = a->agent_type[k];
+            pProg[i] = a->progression[k];
+            pState[i] = a->state[k];
+            pDiag[i] = a->diagnosed[k];
+            pHcw[i] = a->healthcare_worker[k];
+            pLoc[i] = a->location[k];
+            pScal[i] = a->scaling_age_sex[k];
+            const uint32_t b = a->loc_offset[k], e = a->loc_offset[k + 1];
+            std::copy(a->possible_locations + b, a->possible_locations + e, pPLoc.begin() + pOff[i]);
+            std::copy(a->possible_types + b, a->possible_types + e, pPType.begin() + pOff[i]);
+            pOff[i + 1] = pOff[i] + (e - b);
+        }
+        ap.age = pAge.data();
+        ap.sex = pSex.data();
+        ap.precond = pPre.data();
+        ap.agent_type = pType.data();
+        ap.progression = pProg.data();
+        ap.state = pState.data();
+        ap.diagnosed = pDiag.data();
+        ap.healthcare_worker = pHcw.data();
+        ap.location = pLoc.data();
+        ap.scaling_age_sex = pScal.data();
+        ap.loc_offset = pOff.data();
+        ap.possible_locations = pPLoc.data();
+        ap.possible_types = pPType.data();
+        a = &ap;
+    }
     // ---- locations
     Locs& L = c->L;
     L.n = NL;
@@ -408,6 +457,7 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     // ---- agents
     Agents& A = c->A;
     A.n = N;
+    TRY(upload(c, (uint32_t**)&A.id, c->perm.data(), N));
     {
         std::vector<AgentRec> ar(N);
         uint32_t hcw = 0;
@@ -1101,7 +1151,8 @@ int pansim_download_location_sums(pansim_ctx* c, uint32_t v, int64_t* out, size_
     return 0;
 }
 
-int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t* out, size_t n) {
+// A field in device (storage) order.
+static int download_field_stored(pansim_ctx* c, const char* name, uint32_t ts, uint32_t* out, size_t n) {
     if (!c || !name || !out) return PANSIM_E_ARG;
     if (int fr = flush_resolve(c)) return fr;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
@@ -1203,6 +1254,46 @@ int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t
 #undef AGENT_MEMBER
 #undef LOC_MEMBER
     return fail(c, PANSIM_E_ARG, "unknown field " + f);
+}
+
+// A field in input order: agent fields are moved from the storage order
+// back to the input's; the possible-location lists are rebuilt in the
+// input's layout.
+int pansim_download_field(pansim_ctx* c, const char* name, uint32_t ts, uint32_t* out, size_t n) {
+    if (!c || !name || !out) return PANSIM_E_ARG;
+    const std::string f = name;
+    const uint32_t N = c->A.n;
+    const std::vector<uint32_t>& perm = c->perm;
+    if (f == "possibleLocations" || f == "csrBegin") {
+        if (n < (f == "csrBegin" ? (size_t)N : (size_t)c->nCsr)) return fail(c, PANSIM_E_ARG, "output buffer too small");
+        std::vector<uint32_t> cnt(N), beg(N), inBeg(N), pl(c->nCsr);
+        int r = download_field_stored(c, "csrCount", ts, cnt.data(), N);
+        if (!r) r = download_field_stored(c, "csrBegin", ts, beg.data(), N);
+        if (!r) r = download_field_stored(c, "possibleLocations", ts, pl.data(), pl.size());
+        if (r) return r;
+        std::vector<uint32_t> cntIn(N);
+        for (uint32_t i = 0; i < N; ++i) cntIn[perm[i]] = cnt[i];
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < N; ++k) {
+            inBeg[k] = acc;
+            acc += cntIn[k];
+        }
+        if (f == "csrBegin") {
+            std::copy(inBeg.begin(), inBeg.end(), out);
+            return 0;
+        }
+        for (uint32_t i = 0; i < N; ++i) std::copy(pl.begin() + beg[i], pl.begin() + beg[i] + cnt[i], out + inBeg[perm[i]]);
+        return 0;
+    }
+    const int r = download_field_stored(c, name, ts, out, n);
+    if (r) return r;
+    static const char* const locFields[] = {"locQuarantineUntil", "locClosedUntil", "locOpen", "locInfectiousness",
+                                            "locCount"};
+    for (const char* lf : locFields)
+        if (f == lf) return 0;
+    std::vector<uint32_t> stored(out, out + N);
+    for (uint32_t i = 0; i < N; ++i) out[perm[i]] = stored[i];
+    return 0;
 }
 
 int pansim_device_expf(const float* in, float* out, size_t n) {
@@ -1344,10 +1435,12 @@ int pansim_infection_dump_collect(pansim_ctx* c, uint32_t* new_agent, uint32_t* 
         return fail(c, PANSIM_E_ARG, "infection_dump_collect: output arrays missing or too small");
     if (cnt[0]) {
         HIP_OR_FAIL(c, hipMemcpy(new_agent, c->dumpNew, cnt[0] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < cnt[0]; ++k) new_agent[k] = c->perm[new_agent[k]];
         HIP_OR_FAIL(c, hipMemcpy(new_loc, c->dumpNewLoc, cnt[0] * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     if (cnt[1]) {
         HIP_OR_FAIL(c, hipMemcpy(inf_agent, c->dumpInfAgent, cnt[1] * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < cnt[1]; ++k) inf_agent[k] = c->perm[inf_agent[k]];
         HIP_OR_FAIL(c, hipMemcpy(inf_loc, c->dumpInfLoc, cnt[1] * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIP_OR_FAIL(c, hipMemcpy(inf_value, c->dumpInfVal, cnt[1] * sizeof(float), hipMemcpyDeviceToHost));
     }

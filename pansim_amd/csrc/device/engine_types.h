@@ -3,6 +3,8 @@
 // never chases a host-side object.
 //
 // HBM layout, all arrays 256-byte aligned:
+//   agents are stored grouped by schedule (agent type), so that the agents
+//     that move at one step share cache lines; id[] maps back to the input;
 //   agents, per-agent record AgentRec (32 B): the fields that the movement
 //     of one agent reads, so that a mover touches one cache line instead of
 //     one line per field;
@@ -83,6 +85,11 @@ static_assert(sizeof(LocDyn) == 16, "LocDyn is one 16-byte record");
 
 struct Agents {
     uint32_t n;
+    // agent i of the device arrays is agent id[i] of the input: agents are
+    // stored grouped by schedule (agent type), input order within a type;
+    // random streams and tie-breaks are keyed by the input index, so results
+    // do not depend on the storage order
+    const uint32_t* id;
     AgentRec* rec;
     uint8_t* prog;
     uint8_t* sex;

@@ -294,6 +294,8 @@ constexpr uint32_t MOVE_ITEMS = 32;
 constexpr uint32_t MOVE_TILE = MOVE_BLOCK * MOVE_ITEMS;  // agents per block
 constexpr uint32_t MOVE_CTR_SLOTS = 64;
 
+// A pending quarantine write to loc by the agent of input index `agent`; of
+// the writes to one location in one step, the highest input index wins.
 __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, uint32_t loc, uint32_t until) {
     unsigned long long key = ((unsigned long long)agent << 32) | (unsigned long long)until;
     unsigned long long old = atomicMax(&m.L.pendKey[loc], key);
@@ -308,11 +310,11 @@ __device__ __forceinline__ void pend_write(const MoveArgs& m, uint32_t agent, ui
 // workplace, get pending quarantine writes (applied at the end of the step).
 // It runs once, after the rest of the agent's movement routine, so its
 // classroom draw is the routine's last draw.
-__device__ __forceinline__ void quarantine_locations(const MoveArgs& m, uint32_t i, const CsrReg& c, uint32_t until,
+__device__ __forceinline__ void quarantine_locations(const MoveArgs& m, uint32_t key, const CsrReg& c, uint32_t until,
                                                      uint32_t prev, Stream& rs) {
     const pansim_location_types& lt = m.T.lt;
     const uint32_t ts = m.p.timestamp;
-    if (m.p.quarantine_policy > 1 && c.home != NOLOC) pend_write(m, i, c.home, until);
+    if (m.p.quarantine_policy > 1 && c.home != NOLOC) pend_write(m, key, c.home, until);
     if (m.p.quarantine_policy > 2 && prev < ts) {
         uint32_t toClose[2] = {find_loc(m.A, m.L, lt, c, lt.classroom, 0, &rs), find_loc(m.A, m.L, lt, c, lt.work, 0, nullptr)};
 #pragma unroll
@@ -321,7 +323,7 @@ __device__ __forceinline__ void quarantine_locations(const MoveArgs& m, uint32_t
             if (loc == NOLOC) continue;
             const LocRec lr = m.L.rec[loc];
             if ((lr.type == lt.work || lr.type == lt.classroom) && lr.quarUntil < ts && lr.open && lr.closedUntil < ts)
-                pend_write(m, i, loc, until);
+                pend_write(m, key, loc, until);
         }
     }
 }
@@ -351,7 +353,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     // a mover's wake step is this step, except on the curfew step (everyone)
     const uint16_t wake0 = m.curfewStep ? A.wake[i] : (uint16_t)m.stepOfDay;
     uint32_t wake = wake0 == WAKE_NONE ? NEVER : dayStart + wake0;
-    Stream rs(m.seed, i, ts, PH_MOVE, m.replica);
+    const uint32_t key = A.id[i];  // the agent's input index: random streams and tie-breaks
+    Stream rs(m.seed, key, ts, PH_MOVE, m.replica);
     const uint8_t st = r.state;
     const uint8_t w = r.wb;  // T.stWB[st], kept in the record
     // state and variant are fixed during the routine
@@ -617,7 +620,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         wake = wake_dec(ts, sum);
     } while (false);
 
-    if (qLoc) quarantine_locations(m, i, csr, qLocUntil, qLocPrev, rs);
+    if (qLoc) quarantine_locations(m, key, csr, qLocUntil, qLocPrev, rs);
     if (r.flags != r0.flags) A.rec[i].flags = r.flags;
     if (r.diagTs != r0.diagTs) A.rec[i].diagTs = r.diagTs;
     if (r.quarUntil != r0.quarUntil) A.rec[i].quarUntil = r.quarUntil;
@@ -807,7 +810,7 @@ __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, ui
     }
     float ratio = 1.0f - ps_expf(-x);
     if (ratio == 0.0f) return;
-    Stream rs(a.seed, i, a.timestamp, PH_INFECT + v, a.replica);
+    Stream rs(a.seed, a.A.id[i], a.timestamp, PH_INFECT + v, a.replica);
     if (rs.unit() < (double)(ratio * s)) {
         // gotInfected (dynamicPPState.cpp:235-244) + stats (locationList.h:286-291)
         uint8_t fi = (uint8_t)a.T.firstInfected;
@@ -933,7 +936,7 @@ __global__ void k_dummy_move(Agents A, Locs L, Tables T, uint32_t* wait, uint64_
     if (i >= A.n) return;
     uint32_t w = wait[i];
     if (w == 0u) {
-        Stream rs(seed, i, ts, PH_DUMMY_MOVE, replica);
+        Stream rs(seed, A.id[i], ts, PH_DUMMY_MOVE, replica);
         const uint32_t loc = rs.uint_below(L.n);
         w = rs.uint_below(144u / 4u);
         const uint32_t old = A.location[i];
