@@ -50,7 +50,6 @@ and with 1 thread (no sort, as the reference with one thread).
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import os
 import platform
@@ -92,32 +91,6 @@ def parse():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-class Roctx:
-    """roctxProfilerResume / Pause around the timed steps, so that
-    `rocprofv3 --selected-regions` records exactly them; no-ops when the
-    library is absent or no profiler is attached."""
-
-    def __init__(self):
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so", "/opt/rocm/lib/librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            for f in ("roctxProfilerResume", "roctxProfilerPause"):
-                getattr(self.lib, f).argtypes = [ctypes.c_uint64]
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
 
 
 def copy_probe(mib):
@@ -300,11 +273,9 @@ def clock(ts):
     return f"day {ts // STEPS_PER_DAY} {m // 60:02d}:{m % 60:02d}"
 
 
-def steps_timed(sim, n, roctx=None):
+def steps_timed(sim, n):
     """n steps between two synchronisations; returns the wall seconds."""
     sim.sync()
-    if roctx:
-        roctx.resume()
     t0 = time.perf_counter()
     done = 0
     while done < n:
@@ -315,8 +286,6 @@ def steps_timed(sim, n, roctx=None):
             log(f"  {done}/{n} steps")
     sim.sync()
     dt = time.perf_counter() - t0
-    if roctx:
-        roctx.pause()
     return dt
 
 
@@ -342,8 +311,6 @@ def main():
         sys.exit(2)
     import pansim_amd
 
-    roctx = Roctx()
-    roctx.pause()  # only the timed steps are profiled under --selected-regions
     total = a.weeks * 7 * STEPS_PER_DAY
     K, W = a.steps, a.warmup
     args = pansim_amd.synthetic_args(a.agents, a.weeks, a.seed, 1)
@@ -374,7 +341,7 @@ def main():
     # timed steps: one HIP event pair around every movement launch
     sim.movement_counters(reset=True)
     sim.enable_timing(True, movement_only=True)
-    elapsed = steps_timed(sim, K, roctx)
+    elapsed = steps_timed(sim, K)
     n_move, move_ms, _, _ = sim.timing_summary()
     sim.enable_timing(False)
     evaluated, relocated = sim.movement_counters()

@@ -2,8 +2,7 @@
 the full size of the throughput measurement: the 10M-agent synthetic
 household population (4,603,331 locations, all policies on), after a
 daytime step of day 1, one step after the second midnight block, at the
-activity change of 17:50 on day 2 (3.3M agents re-evaluated), one step
-after the last midnight block of the four weeks and at their last step.  The oracle
+evening activity changes of day 2 (17:50), one step after the last midnight block of the four weeks and at their last step.  The oracle
 does not run at this size here: a small oracle population built from the
 same input files supplies the state table and the variant multipliers, and
 the agents' possible-location lists are the ones held on the device."""
