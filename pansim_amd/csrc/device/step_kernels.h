@@ -166,21 +166,19 @@ struct CsrReg {
 };
 
 // Agent i's list (record r): the types of a short list come from its
-// 16-byte record in A.types16, one aligned load.
+// 16-byte record in A.types16 (zero for a list of more than CSR_REG_MAX
+// entries), one aligned load that does not wait for r.
 __device__ __forceinline__ CsrReg load_csr(const Agents& A, uint32_t i, const AgentRec& r) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u t = *reinterpret_cast<const v4u*>(A.types16 + 4 * (size_t)i);
     CsrReg c;
     c.b = r.csrBegin;
     c.n = r.csrCount;
     c.home = r.home;
-    c.t[0] = c.t[1] = c.t[2] = c.t[3] = 0u;
-    if (c.n - 1u < CSR_REG_MAX) {  // 1..16 entries
-        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-        const v4u t = *reinterpret_cast<const v4u*>(A.types16 + 4 * (size_t)i);
-        c.t[0] = t.x;
-        c.t[1] = t.y;
-        c.t[2] = t.z;
-        c.t[3] = t.w;
-    }
+    c.t[0] = t.x;
+    c.t[1] = t.y;
+    c.t[2] = t.z;
+    c.t[3] = t.w;
     return c;
 }
 
@@ -357,12 +355,23 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     Stream rs(m.seed, key, ts, PH_MOVE, m.replica);
     const uint8_t st = r.state;
     const uint8_t w = r.wb;  // T.stWB[st], kept in the record
+    // the schedule window of this step (agentType.cpp:13-15, precomputed per
+    // step of the day) and its first and following events, loaded here, ahead
+    // of the branches below, so that their latency overlaps the agent's other
+    // loads (a dead agent, w = 4, has no window: entry 0 stands in, unused)
+    const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff +
+                             (w <= 3u ? 7u * w + m.p.day_of_week + agentType * 28u : 0u)];
+    pansim_event evFirst = {}, evNext = {};
+    if (win.x >= 0) evFirst = T.ev[win.x];
+    if (win.y >= 0 && (uint32_t)win.y < m.nEvents) evNext = T.ev[win.y];
     // state and variant are fixed during the routine
     const float xInf = T.stInf[st] * T.varMul[r.variant];
     const bool infectedState = T.stInfected[st] != 0;
-    // the last location record read (movement writes no location record)
+    // the last location record read (movement writes no location record);
+    // it starts as the record of a missing location (NOLOC), which a lookup
+    // of a type the agent has no entry for returns
     uint32_t lastL = NOLOC;
-    LocRec lastR;
+    LocRec lastR = lrec(L, NOLOC);
     auto lrec_c = [&](uint32_t l) {
         if (l != lastL) {
             lastR = lrec(L, l);
@@ -486,9 +495,6 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 }
             }
         }
-        // schedule window (agentType.cpp:13-15), precomputed per step of the day
-        const uint32_t off = 7u * w + m.p.day_of_week + agentType * 28u;
-        const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff + off];
         const int ab = win.x, ae = win.y;
         uint32_t newType = NOLOC;
         if (ab == -1 && ae == -1) {  // Case 1
@@ -501,11 +507,11 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             uint32_t picked = 0;
             TD dur;
             if (nPot == 1) {
-                newType = T.ev[ab].loc_type;
-                dur = ev_dur(T.ev[ab]);
+                newType = evFirst.loc_type;
+                dur = ev_dur(evFirst);
             } else {
                 double r = rs.real(1.0);
-                double thr = T.ev[ab].chance;
+                double thr = evFirst.chance;
                 uint32_t idx = 0;
                 while (r > thr && idx < nPot) {
                     idx++;
@@ -513,8 +519,9 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 }
                 if (idx == nPot) idx--;
                 picked = idx;
-                newType = T.ev[ab + idx].loc_type;
-                dur = ev_dur(T.ev[ab + idx]);
+                const pansim_event& pe = idx == 0 ? evFirst : T.ev[ab + idx];
+                newType = pe.loc_type;
+                dur = ev_dur(pe);
             }
             if ((r.flags & F_NO_WORK) && newType == lt.work) {
                 newType = lt.home;
@@ -561,7 +568,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 // ae can equal the slot end: the reference reads the next slot's first
                 // event, or a default event past the end of the array
                 pansim_event ne;
-                if ((uint32_t)ae < m.nEvents) ne = T.ev[ae];
+                if ((uint32_t)ae < m.nEvents) ne = evNext;
                 else {
                     ne.loc_type = 0;
                     ne.chance = -1.0f;
@@ -586,8 +593,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 if (workButClosed || (newType != lt.work && newType != lt.home)) loc = agentHome;
             }
         }
-        if (ab == -1 && ae != -1) {  // Case 3
-            const pansim_event& ne = T.ev[ae];
+        if (ab == -1 && ae != -1) {  // Case 3 (ae is an event of the slot: ae < nEvents)
+            const pansim_event& ne = evNext;
             uint32_t lenSteps = div_dt(m, td_min(td_minus(ev_end(ne), ev_start(ne))));
             uint32_t delay = rs.uint_below(lenSteps);
             sum = div_dt(m, td_min(td_minus(ev_start(ne), simTime))) + delay;
