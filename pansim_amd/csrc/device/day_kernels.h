@@ -131,6 +131,31 @@ __global__ void k_scatter_locations(Agents A, Locs L, Tables T) {
     }
 }
 
+// The sums-only rebuild of the midnight planning: no agent changes location
+// between the day's last movement step and the planning, so the occupancy
+// counts stand; the infectiousness sums (which the midnight passes change
+// through state and variant) are cleared with the visited bits, then
+// refilled by the infectious agents alone.
+__global__ void k_zero_sums(Locs L, uint32_t nVar) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) {
+        L.dyn[l].sum0 = 0;
+        for (uint32_t v = 1; v < nVar; ++v) L.sumX[(size_t)(v - 1) * L.n + l] = 0;
+        if ((l & 31u) == 0) L.visited[l >> 5] = 0u;
+    }
+}
+
+__global__ void k_scatter_sums(Agents A, Locs L, Tables T) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const AgentRec r = A.rec[i];
+    const float x = T.stInf[r.state] * T.varMul[r.variant];
+    if (x == 0.0f) return;
+    const uint32_t loc = A.location[i];
+    atomicAdd((unsigned long long*)loc_sum(L, r.variant, loc), (unsigned long long)fixed32(x));
+    mark_visited(L, loc);
+}
+
 // ------------------------------------------------------------ infectious-list dump
 // (BasicInfection::dumpLocationInfectiousList, infectionPolicies.h:262-441)
 // The locations of the agents infected since the last collection: flagged
@@ -237,11 +262,7 @@ __global__ void k_do_testing(TestArgs a) {
             const uint32_t home = csr.home;
             const uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
             const uint32_t school = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
-            LocRec hr, wr;
-            hr.flag = 0;
-            hr.quarUntil = 0;
-            wr.flag = 0;
-            wr.type = LOC_TYPE_NONE;
+            LocRec hr = lrec(a.L, NOLOC), wr = hr;  // the missing-location record until read
             uint8_t schoolF = 0;
             if (home != NOLOC) hr = a.L.rec[home];
             if (work != NOLOC) wr = a.L.rec[work];
@@ -901,6 +922,7 @@ __global__ void k_plan_locs(PlanArgs a, uint32_t close) {
         const uint32_t lane = threadIdx.x & 63u;
         if (lane == 0u) a.L.special[l >> 5] = (uint32_t)bits;
         if (lane == 32u) a.L.special[l >> 5] = (uint32_t)(bits >> 32);
+        if ((lane & 31u) == 0u) a.L.noWorkBits[l >> 5] = 0u;  // with noWork above
     }
 }
 
@@ -926,8 +948,17 @@ __global__ void k_plan_underage(PlanArgs a, const uint32_t* under, uint32_t n) {
     }
     if (flag) {
         uint32_t h = A.rec[i].home;
-        if (h != NOLOC) a.L.rec[h].noWork = NOWORK_FLAGGED;
+        if (h != NOLOC) {
+            a.L.rec[h].noWork = NOWORK_FLAGGED;
+            atomicOr(&a.L.noWorkBits[h >> 5], 1u << (h & 31u));
+        }
     }
+}
+
+// A home flagged by tonight's planning (an L2-resident bitmap read before the
+// home's location record)
+__device__ __forceinline__ bool nowork_flagged(const Locs& L, uint32_t h) {
+    return h != NOLOC && ((L.noWorkBits[h >> 5] >> (h & 31u)) & 1u) != 0u;
 }
 
 // setNoWorkToday (movementPolicies.h:1137-1158): the lowest eligible agent of
@@ -941,7 +972,7 @@ __global__ void k_plan_nowork_claim(PlanArgs a, const uint32_t* adults, uint32_t
     uint32_t h = A.rec[i].home;
     // 0 = not flagged, NOWORK_FLAGGED = flagged, else (claiming agent's
     // input index + 1)
-    if (h != NOLOC && a.L.rec[h].noWork != 0u) atomicMin(&a.L.rec[h].noWork, A.id[i] + 1u);
+    if (nowork_flagged(a.L, h)) atomicMin(&a.L.rec[h].noWork, A.id[i] + 1u);
 }
 
 __global__ void k_plan_nowork_resolve(PlanArgs a, const uint32_t* adults, uint32_t n) {
@@ -950,7 +981,7 @@ __global__ void k_plan_nowork_resolve(PlanArgs a, const uint32_t* adults, uint32
     if (j >= n) return;
     const uint32_t i = adults[j];
     uint32_t h = A.rec[i].home;
-    if (h != NOLOC && a.L.rec[h].noWork == A.id[i] + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
+    if (nowork_flagged(a.L, h) && a.L.rec[h].noWork == A.id[i] + 1u) A.rec[i].flags = (uint8_t)(A.rec[i].flags | F_NO_WORK);
 }
 
 // ------------------------------------------------------------ closures

@@ -442,6 +442,8 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     // every location special until the first midnight planning computes the bits
     TRY(dalloc(c, &L.special, ((size_t)NL + 31) / 32));
     HIP_OR_FAIL(c, hipMemsetAsync(L.special, 0xFF, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
+    TRY(dalloc(c, &L.noWorkBits, ((size_t)NL + 31) / 32));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.noWorkBits, 0, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &L.pendKey, NL));
     {
         std::vector<float> ones(HAZARD_TYPES, 1.0f);
@@ -801,7 +803,11 @@ int pansim_plan_locations(pansim_ctx* c, const pansim_step_params* p) {
     }
     int r = check_launch(c, "plan_locations");
     if (r) return r;
-    return rebuild_locations(c);
+    // the day's last movement step left every agent where it is now: the
+    // counts stand, the sums are rebuilt
+    hipLaunchKernelGGL(k_zero_sums, dim3(persistent_blocks(c)), dim3(256), 0, c->stream, c->L, c->T.nVar);
+    hipLaunchKernelGGL(k_scatter_sums, dim3(blocks_for(c->A.n)), dim3(256), 0, c->stream, c->A, c->L, c->T);
+    return check_launch(c, "rebuild_sums");
 }
 
 int pansim_testing_day(pansim_ctx* c, uint32_t ts, uint32_t dt, const double probs[6], uint32_t delay, uint32_t qlen,

@@ -131,6 +131,7 @@ struct Locs {
     long long* sumX;                // [V-1][n]: sums of variants 1..V-1
     uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
     uint32_t* special;              // [(n + 31) / 32] bit l clear: l is open, not closed and not quarantined all day
+    uint32_t* noWorkBits;           // [(n + 31) / 32] bit l: home l was flagged for child care by tonight's planning
     unsigned long long* pendKey;    // [n]
     const uint32_t* schools;
     const uint32_t* classrooms;
