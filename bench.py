@@ -233,7 +233,8 @@ def gradient_measure(a):
     """One evaluation of the expected cumulative infections and their
     gradient with respect to k and the location-type multipliers: the
     replicas (seed_i = i) run as one batch for the whole horizon with hazard
-    accumulation on; the populations are built before the timed region."""
+    accumulation on, one hipGraph replay per simulated day; the populations
+    are built before the timed region."""
     import numpy as np
     import pansim_amd
 
@@ -249,7 +250,7 @@ def gradient_measure(a):
     for v in views:
         v.set_location_type_multipliers(m)
         v.hazard_enable(True)
-    ens.steps(steps)
+    ens.days(a.gradient_weeks * 7)  # one replay of the day graph per simulated day
     ens.sync()
     J = dk = 0.0
     for v in views:

@@ -21,6 +21,17 @@ import pansim_amd
 N_TYPES = 256  # multipliers are indexed by location-type id
 
 
+def _steps_per_day(args):
+    """Steps per simulated day of a command line (-t / --deltat minutes, 10 by default)."""
+    a = [str(x) for x in args]
+    for j, x in enumerate(a):
+        if x in ("-t", "--deltat") and j + 1 < len(a):
+            return 1440 // int(a[j + 1])
+        if x.startswith("--deltat="):
+            return 1440 // int(x.split("=", 1)[1])
+    return 144
+
+
 def _with_k(args: Sequence[str], k: float) -> List[str]:
     out, skip = [], False
     for a in args:
@@ -50,7 +61,11 @@ def run_hazard(replica_args: Sequence[Sequence[str]], k: float, m: np.ndarray, s
         for v in views:
             v.set_location_type_multipliers(m)
             v.hazard_enable(True)
-        sims[0].steps(steps)
+        spd = _steps_per_day(reps[0])
+        if len(views) > 1 and steps % spd == 0:
+            sims[0].days(steps // spd)  # one replay of the batch's day graph per simulated day
+        else:
+            sims[0].steps(steps)
         J, dk, dm = 0.0, 0.0, np.zeros(N_TYPES)
         for v in views:
             j, a, b = v.hazard()

@@ -36,6 +36,19 @@ PS_INLINE float ps_floorf(float x) {
     return (t > x) ? t - 1.0f : t;
 }
 
+// exp(r) - 1 for |r| <= ln2/2: r + r^2*q(r) with
+// q(r) = 1/2 + r/6 + r^2/24 + r^3/120 + r^4/720 + r^5/5040 + r^6/40320
+PS_INLINE float expm1_poly(float r) {
+    float q = 2.48015873015873e-05f;
+    q = q * r + 1.98412698412698e-04f;
+    q = q * r + 1.38888888888889e-03f;
+    q = q * r + 8.33333333333333e-03f;
+    q = q * r + 4.16666666666667e-02f;
+    q = q * r + 1.66666666666667e-01f;
+    q = q * r + 0.5f;
+    return r + (r * r) * q;
+}
+
 PS_INLINE float ps_expf(float x) {
     if (x != x) return x;
     if (x > 88.72283935546875f) return bits_to_f32(0x7f800000u);
@@ -46,15 +59,7 @@ PS_INLINE float ps_expf(float x) {
     float kf = ps_floorf(x * INV_LN2 + 0.5f);
     int k = (int)kf;
     float r = (x - kf * LN2_HI) - kf * LN2_LO;
-    // q(r) = 1/2 + r/6 + r^2/24 + r^3/120 + r^4/720 + r^5/5040 + r^6/40320
-    float q = 2.48015873015873e-05f;
-    q = q * r + 1.98412698412698e-04f;
-    q = q * r + 1.38888888888889e-03f;
-    q = q * r + 8.33333333333333e-03f;
-    q = q * r + 4.16666666666667e-02f;
-    q = q * r + 1.66666666666667e-01f;
-    q = q * r + 0.5f;
-    float p = 1.0f + (r + (r * r) * q);
+    float p = 1.0f + expm1_poly(r);
     if (k > 127) return (p * pow2f(127)) * 2.0f;
     if (k < -126) return (p * pow2f(k + 64)) * pow2f(-64);
     return p * pow2f(k);
@@ -99,6 +104,16 @@ PS_INLINE double ps_log(double x) {
     double hfsq = 0.5 * f * f;
     double de = (double)e;
     return de * LN2_HI + (f - (hfsq - (s * (hfsq + R) + de * LN2_LO)));
+}
+
+// ps_expm1f: exp(x) - 1 without the cancellation of ps_expf(x) - 1 near 0.
+// For |x| <= ln2/2 the reduction of ps_expf is the identity (k = 0, r = x)
+// and the polynomial alone is exp(x) - 1 (Taylor to x^8: truncation below
+// 2^-31 relative, then a few roundings); elsewhere |exp(x) - 1| >= 0.29 and
+// ps_expf(x) - 1 loses at most a few ulp.
+PS_INLINE float ps_expm1f(float x) {
+    if (x >= -0.34657359f && x <= 0.34657359f) return expm1_poly(x);
+    return ps_expf(x) - 1.0f;
 }
 
 }  // namespace ps

@@ -419,11 +419,12 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     {
         std::vector<LocRec> lr(NL);
         std::vector<LocDyn> ld(NL);
+        std::vector<uint8_t> t8(NL);
         for (uint32_t k = 0; k < NL; ++k) {
             lr[k].quarUntil = 0;
             lr[k].closedUntil = 0;
             lr[k].noWork = 0;
-            lr[k].type = (uint8_t)l->type[k];
+            lr[k].type = t8[k] = (uint8_t)l->type[k];
             lr[k].open = l->open[k];
             lr[k].essential = l->essential[k];
             lr[k].flag = 0;
@@ -434,6 +435,7 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
         TRY(upload(c, &L.rec, lr.data(), NL));
         TRY(upload(c, &L.dyn, ld.data(), NL));
         TRY(upload(c, &L.infRaw, l->infectiousness, NL));
+        TRY(upload(c, (uint8_t**)&L.type8, t8.data(), NL));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     }
     TRY(dalloc(c, &L.sumX, (size_t)(V - 1) * NL));

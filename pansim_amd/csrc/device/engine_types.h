@@ -128,6 +128,7 @@ struct Locs {
     LocRec* rec;
     LocDyn* dyn;
     float* infRaw;                  // [n] infectiousness (input, scaled by the closure rules)
+    const uint8_t* type8;           // [n] location type (the type of LocRec, 1 B per location: cache resident)
     long long* sumX;                // [V-1][n]: sums of variants 1..V-1
     uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
     uint32_t* special;              // [(n + 31) / 32] bit l clear: l is open, not closed and not quarantined all day

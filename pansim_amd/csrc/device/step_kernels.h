@@ -779,25 +779,32 @@ static_assert(INFECT_BLOCK % 64 == 0, "the infection pass maps 64-agent runs to 
 // The expected-infection hazard of one susceptible agent at a location with
 // infectious visitors, s * (1 - exp(-x)) with x = k * density * inf * m_type *
 // season * dt, and its derivatives with respect to k and m_type, with the
-// trajectory held fixed (FP64, accumulated into LDS slots of the block).
-__device__ __forceinline__ void hazard_terms(double* sh, float k, float density, float inf, float m, uint32_t type,
-                                             float season, float dt, float x, float s, float probeH,
+// trajectory held fixed (FP32 terms, FP64 sums).  The terms every candidate
+// adds to (J, dJ/dk and the four probe sums) are
+// kept in registers, acc[0..5], and reduced once per wave at the end of the
+// tile; the term of its location's type goes to the block's LDS slot.
+constexpr uint32_t HAZARD_ACC = 6;
+__device__ __forceinline__ void hazard_terms(double* acc, double* sh, float k, float density, float inf, float m,
+                                             uint32_t type, float season, float dt, float x, float s, float probeH,
                                              uint32_t probeType) {
-    const double ex = exp(-(double)x);
-    const double w = (double)s * ex;
-    atomicAdd(&sh[0], (double)s * (1.0 - ex));
-    atomicAdd(&sh[1], w * ((double)density * (double)inf * (double)m * (double)season * (double)dt));
-    atomicAdd(&sh[2 + type], w * ((double)k * (double)density * (double)inf * (double)season * (double)dt));
+    // each term in FP32 (1 - e^-x through expm1: no cancellation for small
+    // x), accumulated in FP64
+    const float em = ps_expm1f(-x);  // e^-x - 1
+    const float w = s * (1.0f + em);  // s e^-x
+    acc[0] += (double)(s * -em);
+    acc[1] += (double)(w * (density * inf * m * season * dt));
+    atomicAdd(&sh[2 + type], (double)(w * (k * density * inf * season * dt)));
     if (probeH > 0.0f) {
         // x is linear in k and in the multiplier of its location's type: J at
-        // a parameter scaled by (1 +- h) changes by s (e^-x - e^-x(1 +- h))
-        const double dp = (double)s * (ex - exp(-(double)x * (1.0 + (double)probeH)));
-        const double dm = (double)s * (ex - exp(-(double)x * (1.0 - (double)probeH)));
-        atomicAdd(&sh[HAZARD_PROBE0], dp);
-        atomicAdd(&sh[HAZARD_PROBE0 + 1], dm);
+        // a parameter scaled by (1 +- h) changes by s e^-x (1 - e^-+xh)
+        const float xh = x * probeH;
+        const float dp = -w * ps_expm1f(-xh);
+        const float dm = -w * ps_expm1f(xh);
+        acc[2] += (double)dp;
+        acc[3] += (double)dm;
         if (type == probeType) {
-            atomicAdd(&sh[HAZARD_PROBE0 + 2], dp);
-            atomicAdd(&sh[HAZARD_PROBE0 + 3], dm);
+            acc[4] += (double)dp;
+            acc[5] += (double)dm;
         }
     }
 }
@@ -805,15 +812,18 @@ __device__ __forceinline__ void hazard_terms(double* sh, float k, float density,
 // infm: the location's infectiousness times its type's multiplier (LocDyn).
 template <bool HAZARD>
 __device__ __forceinline__ void infect_agent(const InfectArgs& a, uint32_t i, uint32_t v, uint32_t loc, long long e,
-                                             uint32_t count, float infm, float s, double* sh) {
+                                             uint32_t count, float infm, float s, double* acc, double* sh) {
     const uint32_t n = a.A.n;
     float num = (float)((double)e * (1.0 / 4294967296.0));
     float density = num / (float)count;
     const float x = a.k * density * infm * a.season * a.dt;
     if (HAZARD) {
-        const uint32_t type = a.L.rec[loc].type;
-        hazard_terms(sh, a.k, density, a.L.infRaw[loc], a.typeMul[type], type, a.season, a.dt, x, s, a.probeH,
-                     a.probeType);
+        const uint32_t type = a.L.type8[loc];
+        const float m = a.typeMul[type];
+        // infm = inf * m (the LocDyn value loaded with the sum): the raw
+        // infectiousness without a second gather, unless m is 0
+        const float inf = m != 0.0f ? infm / m : a.L.infRaw[loc];
+        hazard_terms(acc, sh, a.k, density, inf, m, type, a.season, a.dt, x, s, a.probeH, a.probeType);
     }
     float ratio = 1.0f - ps_expf(-x);
     if (ratio == 0.0f) return;
@@ -870,6 +880,9 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
     if (a.resolve) resolve_pending(a.L, a.S, a.resolvePar, tile, a.nTiles);
     __shared__ double shHazard[HAZARD ? HAZARD_SLOTS : 1];
     double* sh = shHazard;
+    double acc[HAZARD ? HAZARD_ACC : 1];
+#pragma unroll
+    for (uint32_t j = 0; j < (HAZARD ? HAZARD_ACC : 1); ++j) acc[j] = 0.0;
     if (HAZARD) {
         for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x) shHazard[j] = 0.0;
         __syncthreads();
@@ -911,10 +924,19 @@ __device__ __forceinline__ void infect_tile(const InfectArgs& a, uint32_t tile) 
             const uint32_t i = base + 64u * k;
             const uint32_t l = loc[k];
             const LocDyn d = a.L.dyn[l];
-            infect_agent<HAZARD>(a, i, v, l, e[k], d.count, d.inf, s[k], sh);
+            infect_agent<HAZARD>(a, i, v, l, e[k], d.count, d.inf, s[k], acc, sh);
         }
     }
     if (HAZARD) {
+        // the register sums: one LDS add per wave
+        const uint32_t slot[HAZARD_ACC] = {0u, 1u, HAZARD_PROBE0, HAZARD_PROBE0 + 1u, HAZARD_PROBE0 + 2u,
+                                           HAZARD_PROBE0 + 3u};
+#pragma unroll
+        for (uint32_t j = 0; j < (HAZARD ? HAZARD_ACC : 1); ++j) {
+            double v = acc[j];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63u) == 0u && v != 0.0) atomicAdd(&sh[slot[j]], v);
+        }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < HAZARD_SLOTS; j += blockDim.x)
             if (shHazard[j] != 0.0) atomicAdd(&a.hazard[j], shHazard[j]);
