@@ -205,9 +205,19 @@ struct TestArgs {
     unsigned long long* counters;  // tests, positive now, positive in the last day
 };
 
+// The testing flags live in a bitmap (1 bit per location, L2 resident): the
+// testing pass reads the flags of every agent's home, workplace and school.
+__device__ __forceinline__ bool test_flag(const Locs& L, uint32_t l) {
+    return ((L.flagBits[l >> 5] >> (l & 31u)) & 1u) != 0u;
+}
+__device__ __forceinline__ void set_test_flag(const Locs& L, uint32_t l) {
+    atomicOr(&L.flagBits[l >> 5], 1u << (l & 31u));
+}
+
 __global__ void k_clear_flags(Locs L) {
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < L.n; l += stride) L.rec[l].flag = 0;
+    const uint32_t words = (L.n + 31u) / 32u;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) L.flagBits[w] = 0u;
 }
 
 // DetailedTestingOps::flagLocations (testingPolicies.h:59-148)
@@ -224,15 +234,15 @@ __global__ void k_flag_locations(TestArgs a) {
     };
     const CsrReg csr = load_csr(a.A, i, a.A.rec[i]);
     uint32_t home = csr.home;
-    if (home != NOLOC) a.L.rec[home].flag = 1;
+    if (home != NOLOC) set_test_flag(a.L, home);
     uint32_t work = find_loc(a.A, a.L, lt, csr, lt.work, 0, nullptr);
-    if (work != NOLOC && ok(work)) a.L.rec[work].flag = 1;
+    if (work != NOLOC && ok(work)) set_test_flag(a.L, work);
     uint32_t school = find_loc(a.A, a.L, lt, csr, lt.school, 0, nullptr);
     if (school != NOLOC && ok(school)) {
-        a.L.rec[school].flag = 1;
+        set_test_flag(a.L, school);
         Stream rs(a.seed, a.A.id[i], a.ts, PH_TEST_FLAG, a.replica);
         uint32_t cls = find_loc(a.A, a.L, lt, csr, lt.classroom, 0, &rs);
-        if (cls != NOLOC && ok(cls)) a.L.rec[cls].flag = 1;
+        if (cls != NOLOC && ok(cls)) set_test_flag(a.L, cls);
     }
 }
 
@@ -262,16 +272,13 @@ __global__ void k_do_testing(TestArgs a) {
             const uint32_t home = csr.home;
             const uint32_t work = find_loc(A, a.L, lt, csr, lt.work, 0, nullptr);
             const uint32_t school = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr);
-            LocRec hr = lrec(a.L, NOLOC), wr = hr;  // the missing-location record until read
-            uint8_t schoolF = 0;
-            if (home != NOLOC) hr = a.L.rec[home];
-            if (work != NOLOC) wr = a.L.rec[work];
-            if (school != NOLOC) schoolF = a.L.rec[school].flag;
-            const bool homeFlag = hr.flag != 0;
-            const bool workFlag = wr.flag != 0;
+            // flags from the bitmap, the workplace's type from the type bytes
+            const bool homeFlag = home != NOLOC && test_flag(a.L, home);
+            const bool workFlag = work != NOLOC && test_flag(a.L, work);
+            const uint32_t workType = work != NOLOC ? (uint32_t)a.L.type8[work] : (uint32_t)LOC_TYPE_NONE;
             bool schoolFlag = false, classFlag = false;
             if (school != NOLOC) {
-                schoolFlag = schoolF != 0;
+                schoolFlag = test_flag(a.L, school);
                 uint32_t cls = find_loc(A, a.L, lt, csr, lt.classroom, 0, &rs);
                 if (cls != NOLOC) classFlag = true;
             }
@@ -280,8 +287,8 @@ __global__ void k_do_testing(TestArgs a) {
             prob += (double)workFlag * a.pWork;
             prob += (double)schoolFlag * a.pSchool;
             prob += (double)classFlag * 3.0 * a.pSchool;
-            if (work != NOLOC && (wr.type == lt.doctor || wr.type == lt.hospital)) prob += a.pHospital;
-            if (work != NOLOC && wr.type == lt.nursery_home) prob += a.pNursery;
+            if (work != NOLOC && (workType == lt.doctor || workType == lt.hospital)) prob += a.pHospital;
+            if (work != NOLOC && workType == lt.nursery_home) prob += a.pNursery;
             if (r.hospTs <= ts && r.hospUntil > ts) prob += a.pHospital;
             if (prob > 1.0 || rs.real(1.0) < prob) {
                 A.lastTest[i] = ts;
@@ -294,7 +301,7 @@ __global__ void k_do_testing(TestArgs a) {
                         diagTs = ts;
                     }
                 } else {
-                    if (r.quarUntil > ts && (home != NOLOC && hr.quarUntil < ts)) {
+                    if (r.quarUntil > ts && (home != NOLOC && a.L.rec[home].quarUntil < ts)) {
                         if (A.daysInQ[i] > 0) A.daysInQ[i] -= (r.quarUntil - ts) / day;
                         A.rec[i].quarUntil = ts;
                     }

@@ -446,6 +446,8 @@ int pansim_create(pansim_ctx** out, const pansim_dims* d, const pansim_tables* t
     HIP_OR_FAIL(c, hipMemsetAsync(L.special, 0xFF, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &L.noWorkBits, ((size_t)NL + 31) / 32));
     HIP_OR_FAIL(c, hipMemsetAsync(L.noWorkBits, 0, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
+    TRY(dalloc(c, &L.flagBits, ((size_t)NL + 31) / 32));
+    HIP_OR_FAIL(c, hipMemsetAsync(L.flagBits, 0, ((size_t)NL + 31) / 32 * sizeof(uint32_t), c->stream));
     TRY(dalloc(c, &L.pendKey, NL));
     {
         std::vector<float> ones(HAZARD_TYPES, 1.0f);

@@ -69,7 +69,7 @@ struct __align__(16) LocRec {
     uint8_t type;
     uint8_t open;
     uint8_t essential;
-    uint8_t flag;          // testing flag
+    uint8_t flag;          // unused (the testing flags are the bits of Locs::flagBits)
 };
 static_assert(sizeof(LocRec) == 16, "LocRec is one 16-byte record");
 
@@ -133,6 +133,7 @@ struct Locs {
     uint32_t* visited;              // [(n + 31) / 32] bit l: an infectious agent was at l today (superset of sum != 0)
     uint32_t* special;              // [(n + 31) / 32] bit l clear: l is open, not closed and not quarantined all day
     uint32_t* noWorkBits;           // [(n + 31) / 32] bit l: home l was flagged for child care by tonight's planning
+    uint32_t* flagBits;             // [(n + 31) / 32] bit l: the testing flag of l (flagLocations)
     unsigned long long* pendKey;    // [n]
     const uint32_t* schools;
     const uint32_t* classrooms;
