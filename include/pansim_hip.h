@@ -127,7 +127,10 @@ typedef struct {
 
 /* Creates a context on the current HIP device, allocates all device arrays
  * and uploads the tables, locations and agents (copied; the caller keeps
- * ownership).  Draws every agent's initial daysBeforeNextState on the device
+ * ownership).  On the device the agents are stored grouped by agent type
+ * (input order within a type); every random stream and tie-break is keyed by
+ * the agent's input index, and every agent index or per-agent array this
+ * interface returns is in input order, so the storage order is invisible.  Draws every agent's initial daysBeforeNextState on the device
  * (DynamicPPState(name, progressionID), dynamicPPState.cpp:279-285) and sets
  * the initial AgentStats (agentsList.h:179-194).  quarantine_policy and
  * quarantine_length are needed for those initial stats.
@@ -276,8 +279,8 @@ int pansim_expose_mutation(pansim_ctx* ctx, uint32_t timestamp, uint32_t variant
 int pansim_upload_schedules(pansim_ctx* ctx, const uint32_t* event_offset, uint32_t n_offsets, const pansim_event* events,
                             uint32_t n_events);
 
-/* Downloads a named per-agent or per-location field as uint32 (floats as
- * raw bits): "state", "variant", "days", "location", "stepsUntilMove",
+/* Downloads a named per-agent (in input order) or per-location field as
+ * uint32 (floats as raw bits): "state", "variant", "days", "location", "stepsUntilMove",
  * "diagnosed", "quarantined", "stayedHome", "noWorkAgent", "lastTest",
  * "susceptible0", AgentStats members, "vaccinationRound", "locQuarantineUntil",
  * "locClosedUntil", "locOpen", "locInfectiousness", "locCount", and the
