@@ -95,6 +95,8 @@ struct Stream {
     }
     // uniform double in [0, max)
     PS_INLINE double real(double max) { return max * unit(); }
+    // a draw whose value is not used: the counter moves on, no block is computed
+    PS_INLINE void skip() { ++draw; }
     // uniform integer in [0, max): multiply-shift on word 2.  max == 0 returns
     // 0 without consuming a draw, as randomUnsigned(0) does.
     PS_INLINE uint32_t uint_below(uint32_t max) {

@@ -423,6 +423,12 @@ __global__ void k_immunization(ImmArgs a) {
     const Agents& A = a.A;
     if (i >= A.n) return;
     if (A.rec[i].state != 0) return;
+    const uint8_t infC = A.infCount[i], immC = A.immCount[i];
+    // never infected nor immunized: susceptibility 1 for every variant, which
+    // it already is when state 0 is a susceptible state (set with that state
+    // when the agent was created, and written since only by an infection or
+    // by the branches below)
+    if (infC == 0 && immC == 0 && a.T.stSusc[0]) return;
     const uint32_t nVar = a.T.nVar;
     const uint32_t ts = a.ts == 0 ? 1u : a.ts;
     const uint32_t week = 7u * 24u * 60u / a.dt;
@@ -430,7 +436,6 @@ __global__ void k_immunization(ImmArgs a) {
     for (uint32_t v = 0; v < nVar; ++v) susc[v] = 1.0f;
     const uint32_t wImm = (ts - A.immTs[i]) / week;
     const uint32_t wInf = (ts - A.infTs[i]) / week;
-    const uint8_t infC = A.infCount[i], immC = A.immCount[i];
     const uint32_t prior = A.statVar[i];
     float scal = A.scal[i];
     auto lin = [&](int e, int row, uint32_t weeks) -> float {
@@ -715,7 +720,7 @@ __global__ void k_other_disease(Agents A, Tables T, uint64_t seed, uint32_t repl
     }
     uint8_t w = T.stWB[st];
     if (w == 4 || w == 3 || ts < A.rec[i].hospUntil) return;
-    (void)rs.real(1.0);  // sudden death: disabled in the reference, draw still made
+    rs.skip();  // sudden death: disabled in the reference, its draw still made (and unused)
     const int multDay[4] = {99, 99 + 365, 99 + 2 * 365, 99 + 3 * 365};
     const double mult[4] = {0.82, 0.67, 0.9, 0.9};
     int day = (int)(ts / (24u * 60u / dt));
