@@ -38,15 +38,20 @@
 // Philox known-answer vectors, exp/log within one ulp of correctly rounded
 // values, the truncated-geometric mean, the geometric distribution, the
 // next-state shares of selectNext against the progression files' chances,
-// seasonality by date properties and exact cosine points, the testing
-// pass one probability term at a time, the first-dose protection rule of
-// the immunization pass, the three closure conditions, the Masks,
-// ExposeToMutation and Curfew actions, the structure of the non-COVID
-// hospital stays, and the invariants of movement, location sums and
-// statistics.  Parity unpinned (no reference test, golden vector or closed
-// form covers them; they are checked only as the GPU path's oracle): the
-// HolidayMode and ReduceMovement actions, otherDisease's probability table,
-// and the immunization pass after an infection or a booster.
+// seasonality by date properties and exact cosine points, hand-computed
+// movement trajectories and planning midnights on schedules written in the
+// tests (with the event and re-pick shares), the input files read
+// independently in plain Python, the testing pass one probability term at a
+// time, every case of the immunization pass (first dose, after an infection,
+// infection plus doses, boosters), the three closure conditions, the Masks,
+// ExposeToMutation, Curfew, HolidayMode and ReduceMovement actions, the
+// non-COVID hospital stays against otherDisease's probability table, the
+// statistics vector recounted from the fields, and the brute-force recount
+// of the location sums.  The movement and planning pins and the full-size
+// property checks are also run against a planted mistake (the codes below,
+// and wrong reported fields) and seen to fail.  Parity unpinned: the spread
+// of outcomes over replicas (the ensemble bands), which needs a runnable
+// reference.
 #include <stdint.h>
 
 #include <algorithm>
