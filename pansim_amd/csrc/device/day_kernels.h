@@ -863,7 +863,11 @@ __global__ void k_plan_school(PlanArgs a) {
     }
     if (counter > 1) {
         q[school].quarUntil = mx;
-        for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) q[a.L.classrooms[c]].quarUntil = mx;
+        mark_special(a.L, school);
+        for (uint32_t c = a.L.classOff[s]; c < a.L.classOff[s + 1]; ++c) {
+            q[a.L.classrooms[c]].quarUntil = mx;
+            mark_special(a.L, a.L.classrooms[c]);
+        }
     }
 }
 
@@ -887,11 +891,15 @@ __global__ void k_plan_agents(PlanArgs a, uint32_t schoolwork) {
         uint32_t wl = find_loc(A, a.L, lt, csr, lt.work, 0, &rs, true);
         uint32_t sl = find_loc(A, a.L, lt, csr, lt.school, 0, nullptr, true);
         uint32_t cl = find_loc(A, a.L, lt, csr, lt.classroom, 0, nullptr, true);
-        // the three quarantine times loaded together, then the first that applies
+        // the three quarantine times loaded together, then the first that
+        // applies; a location whose special bit is clear was last quarantined
+        // until at most the first step of the day that ends now (every
+        // quarantine write since, tonight's school quarantines included, sets
+        // the bit), so its record is not read
         const LocRec* q = a.L.rec;
-        const uint32_t qs = sl != NOLOC ? q[sl].quarUntil : 0u;
-        const uint32_t qc = cl != NOLOC ? q[cl].quarUntil : 0u;
-        const uint32_t qw = wl != NOLOC ? q[wl].quarUntil : 0u;
+        const uint32_t qs = sl != NOLOC && loc_special(a.L, sl) ? q[sl].quarUntil : 0u;
+        const uint32_t qc = cl != NOLOC && loc_special(a.L, cl) ? q[cl].quarUntil : 0u;
+        const uint32_t qw = wl != NOLOC && loc_special(a.L, wl) ? q[wl].quarUntil : 0u;
         uint32_t until = 0;
         if (qs > a.ts) until = qs;
         else if (qc > a.ts) until = qc;
