@@ -106,6 +106,23 @@ def test_parity_no_testing_and_no_closure(flags):
     run_parity(args, [1, 144, 145, 600, 1008])
 
 
+@pytest.mark.parametrize("dt", [5, 15, 30, 60])
+def test_parity_other_time_steps(dt):
+    # the step length (-t) sets the steps per day, the schedule window of
+    # each step, the step count of every stay and quarantine, and the
+    # midnight cadence; checkpoints after the first step, mid-morning, at
+    # and after midnight, mid-day 5 and at the end of two weeks
+    spd = 1440 // dt
+    args = _cfg(n=2500, nl=500, weeks=2, seed=20 + dt, k="0.03", extra=["-t", str(dt)])
+    run_parity(args, [1, spd // 3, spd, spd + 1, 5 * spd + spd // 2, 14 * spd])
+
+
+def test_parity_synthetic_population_15_minute_steps():
+    # the generator's schedule jitter moves events by multiples of the step
+    args = pansim_amd.synthetic_args(6000, 1, 4, 6, extra=["-k", "0.01", "--synthInfected", "0.01", "-t", "15"])
+    run_parity(args, [40, 96, 97, 4 * 96 + 50, 7 * 96])
+
+
 def test_parity_config1_full_size():
     # the CPU-runnable workload at its size: -n 100000 -N 20000 (96,786 agents
     # after the tourists are skipped, 20,001 locations), two weeks, seed 0;
