@@ -6,6 +6,7 @@ quarantine and hospital paths is reached within the run."""
 import json
 import os
 
+import numpy as np
 import pytest
 
 import pansim_amd
@@ -121,6 +122,58 @@ def test_parity_synthetic_population_15_minute_steps():
     # the generator's schedule jitter moves events by multiples of the step
     args = pansim_amd.synthetic_args(6000, 1, 4, 6, extra=["-k", "0.01", "--synthInfected", "0.01", "-t", "15"])
     run_parity(args, [40, 96, 97, 4 * 96 + 50, 7 * 96])
+
+
+def _long_list_world(tmp_path, n=1500, seed=3):
+    """Agents whose possible-location lists hold 20 to 40 entries (two to four
+    of every type the shipped schedules visit), half of them, and short
+    lists for the rest: lists longer than 16 entries take the engine's
+    in-memory lookup, the short ones its register copy, in the same waves."""
+    from tests.mini_world import person, place
+    rng = np.random.default_rng(seed)
+    places, kinds = [], {}
+
+    def add(t, count, essential=0):
+        ids = []
+        for _ in range(count):
+            ids.append(len(places))
+            places.append(place(f"L{len(places)}", t, essential=essential))
+        kinds[t] = ids
+
+    add(2, n // 2)                      # homes
+    add(4, n // 12)                     # workplaces
+    add(3, 6)                           # schools
+    kinds[33] = []                      # classrooms, five per school ("class_school" ids)
+    for j in range(30):
+        kinds[33].append(len(places))
+        places.append(place(f"C{j}_L{kinds[3][j % 6]}", 33))
+    for t in (1, 5, 6, 7, 8, 9, 10, 15, 44):
+        add(t, 25)
+    add(14, 6, essential=1)             # doctors
+    add(12, 2, essential=1)             # hospitals
+    people = []
+    for i in range(n):
+        cls = int(rng.integers(0, 30))
+        locs = [(2, kinds[2][i // 2]), (4, int(rng.choice(kinds[4]))), (3, kinds[3][cls % 6]), (33, kinds[33][cls]),
+                (12, int(rng.choice(kinds[12]))), (14, int(rng.choice(kinds[14])))]
+        per = 2 + int(rng.integers(0, 3)) if i % 2 == 0 else 1
+        for t in (1, 5, 6, 7, 8, 9, 10, 15, 44):
+            for _ in range(per if i % 2 == 0 else int(rng.integers(0, 2))):
+                locs.append((t, int(rng.choice(kinds[t]))))
+        if i % 2 == 0:
+            rng.shuffle(locs)
+        state = "E" if rng.random() < 0.03 else "S"
+        people.append(person(int(rng.integers(1, 9)), [(t, places[l]["ID"]) for t, l in locs], state=state,
+                             age=int(rng.integers(3, 90)), sex="M" if i % 3 else "F"))
+    (tmp_path / "locations.json").write_text(json.dumps({"places": places}))
+    (tmp_path / "agents.json").write_text(json.dumps({"people": people}))
+    return ["-a", str(tmp_path / "agents.json"), "-l", str(tmp_path / "locations.json"), "-w", "2", "-t", "10",
+            "-k", "0.05", "--quarantinePolicy", "3", "--closures", "inputConfigFiles/closureRules.json",
+            "--seed", str(seed)]
+
+
+def test_parity_long_location_lists(tmp_path):
+    run_parity(_long_list_world(tmp_path), [1, 50, 144, 145, 700, 1500, 2016])
 
 
 def test_parity_config1_full_size():
