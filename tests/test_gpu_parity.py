@@ -176,6 +176,14 @@ def test_parity_long_location_lists(tmp_path):
     run_parity(_long_list_world(tmp_path), [1, 50, 144, 145, 700, 1500, 2016])
 
 
+def test_parity_reference_example_population(tmp_path):
+    # the reference's own example input (tests/real_example.py): real
+    # schedules' location types, lists of about 19 entries, 1 % exposed at
+    # the start, two weeks with the shipped closure rules
+    from tests import real_example
+    run_parity(real_example.args(tmp_path), [1, 48, 144, 145, 700, 1500, 2016])
+
+
 def test_parity_config1_full_size():
     # the CPU-runnable workload at its size: -n 100000 -N 20000 (96,786 agents
     # after the tourists are skipped, 20,001 locations), two weeks, seed 0;

@@ -188,3 +188,15 @@ def test_oracle_vaccination_rollout():
         assert (it[boosted] >= 125 * 144).all()
     finally:
         o.close()
+
+
+def test_invariants_on_the_reference_example_population(tmp_path):
+    # the reference's example input (tests/real_example.py): the locations,
+    # occupancy and sums invariants at an activity change and at the end of
+    # a week
+    from tests import real_example
+    o = OracleSim(real_example.args(tmp_path, weeks=1, k="0.002"))
+    for step in (52, 1008 - 52):
+        o.steps(step)
+        _check_locations(o)
+        _check_sums(o)
