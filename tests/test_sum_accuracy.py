@@ -6,8 +6,8 @@ population whose location sizes are those of the 10M-agent workload
 workplaces capped at 2000).  The engine keeps each sum exactly in 2^-32
 fixed point and converts it once to float (the device sums equal the
 oracle's bit for bit, GPU parity tests), so its relative difference from the
-FP64 sum is at most 2^-24; from the reference's FP32 order the north star
-allows 1e-5.  The measured maxima are printed."""
+FP64 sum is at most 2^-24; from the reference's FP32 order the allowed
+relative difference is 1e-5.  The measured maxima are printed."""
 import numpy as np
 
 import pansim_amd
