@@ -355,18 +355,32 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     Stream rs(m.seed, key, ts, PH_MOVE, m.replica);
     const uint8_t st = r.state;
     const uint8_t w = r.wb;  // T.stWB[st], kept in the record
-    // the schedule window of this step (agentType.cpp:13-15, precomputed per
-    // step of the day) and its first and following events, loaded here, ahead
-    // of the branches below, so that their latency overlaps the agent's other
-    // loads (a dead agent, w = 4, has no window: entry 0 stands in, unused)
+    // The loads below depend only on the agent's record and location: they
+    // are issued together, ahead of the branches that use them, and their
+    // values are combined only where used, so that a wave waits for one
+    // round trip where a branch per load would wait for one each.
+    // The schedule window of this step (agentType.cpp:13-15, precomputed per
+    // step of the day; a dead agent, w = 4, has no window: entry 0 stands
+    // in, unused).
     const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff +
                              (w <= 3u ? 7u * w + m.p.day_of_week + agentType * 28u : 0u)];
-    pansim_event evFirst = {}, evNext = {};
-    if (win.x >= 0) evFirst = T.ev[win.x];
-    if (win.y >= 0 && (uint32_t)win.y < m.nEvents) evNext = T.ev[win.y];
     // state and variant are fixed during the routine
-    const float xInf = T.stInf[st] * T.varMul[r.variant];
-    const bool infectedState = T.stInfected[st] != 0;
+    const float stInfV = T.stInf[st], varMulV = T.varMul[r.variant];
+    const uint8_t stInfectedV = T.stInfected[st];
+    // the special-bitmap word of the agent's present location (oldLoc is a
+    // location of this population, below L.n)
+    const uint32_t oldWord = L.special[oldLoc >> 5];
+    // the window's first and following events, read from valid indices
+    // (entry 0 otherwise) and replaced by the empty event when absent, so
+    // that neither load waits for a branch
+    const bool hasFirst = win.x >= 0, hasNext = win.y >= 0 && (uint32_t)win.y < m.nEvents;
+    const pansim_event e0 = T.ev[hasFirst ? (uint32_t)win.x : 0u];
+    const pansim_event e1 = T.ev[hasNext ? (uint32_t)win.y : 0u];
+    const pansim_event none = {};
+    const pansim_event evFirst = hasFirst ? e0 : none, evNext = hasNext ? e1 : none;
+    const float xInf = stInfV * varMulV;
+    const bool infectedState = stInfectedV != 0;
+    const bool oldSpecial = ((oldWord >> (oldLoc & 31u)) & 1u) != 0u;
     // the last location record read (movement writes no location record);
     // it starts as the record of a missing location (NOLOC), which a lookup
     // of a type the agent has no entry for returns
@@ -467,7 +481,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             if (agentHome != NOLOC && loc != agentHome && sum > m.hourSteps) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
         // (a location that is not special is not quarantined today)
-        if (m.p.quarantine_policy > 1 && loc_special(L, loc)) {
+        if (m.p.quarantine_policy > 1 && (loc == oldLoc ? oldSpecial : loc_special(L, loc))) {
             const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
