@@ -393,6 +393,17 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
         }
         return lastR;
     };
+    // the last special bit read (loc_special), starting with the agent's
+    // present location: the checks of one destination share one read
+    uint32_t lastSL = oldLoc;
+    bool lastS = oldSpecial;
+    auto special_c = [&](uint32_t l) {
+        if (l != lastSL) {
+            lastS = loc_special(L, l);
+            lastSL = l;
+        }
+        return lastS;
+    };
 
     // quarantineAgent (movementPolicies.h:171-245) without its location
     // writes: an agent is quarantined at most once per routine, and only a
@@ -481,7 +492,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             if (agentHome != NOLOC && loc != agentHome && sum > m.hourSteps) r.flags &= (uint8_t)~F_STAYED_HOME;
         };
         // (a location that is not special is not quarantined today)
-        if (m.p.quarantine_policy > 1 && (loc == oldLoc ? oldSpecial : loc_special(L, loc))) {
+        if (m.p.quarantine_policy > 1 && special_c(loc)) {
             const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
@@ -546,7 +557,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             bool tooOld = (newType == lt.school || newType == lt.classroom) && r.age >= m.p.school_age_restriction;
             bool lockdown = m.p.lockdown_nonvacc_active && A.immTs[i] == 0;
             auto closedHere = [&](uint32_t l, uint32_t t) {
-                if (t == lt.work || !loc_special(L, l)) return false;
+                if (t == lt.work || !special_c(l)) return false;
                 const LocRec lr = lrec_c(l);
                 return lr.open == 0 || lr.closedUntil > ts;
             };
@@ -622,7 +633,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             }
         }
         // (a location that is not special is not quarantined today)
-        if (m.p.quarantine_policy > 1 && loc_special(L, loc)) {
+        if (m.p.quarantine_policy > 1 && special_c(loc)) {
             const LocRec lr = lrec_c(loc);
             if (ts < lr.quarUntil && quarantinable(lr)) {
                 if (!(r.flags & F_QUARANTINED)) quarantine(lr.quarUntil);
