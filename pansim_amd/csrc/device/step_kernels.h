@@ -538,15 +538,20 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
                 double r = rs.real(1.0);
                 double thr = evFirst.chance;
                 uint32_t idx = 0;
+                // the last event read: the picked one when the loop ends (an
+                // exhausted loop steps back to the last one it read)
+                pansim_event cur = evFirst;
                 while (r > thr && idx < nPot) {
                     idx++;
-                    if (idx < nPot) thr += T.ev[ab + idx].chance;
+                    if (idx < nPot) {
+                        cur = T.ev[ab + idx];
+                        thr += cur.chance;
+                    }
                 }
                 if (idx == nPot) idx--;
                 picked = idx;
-                const pansim_event& pe = idx == 0 ? evFirst : T.ev[ab + idx];
-                newType = pe.loc_type;
-                dur = ev_dur(pe);
+                newType = cur.loc_type;
+                dur = ev_dur(cur);
             }
             if ((r.flags & F_NO_WORK) && newType == lt.work) {
                 newType = lt.home;
