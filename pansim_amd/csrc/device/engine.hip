@@ -50,7 +50,7 @@ struct pansim_ctx {
     uint32_t evOffCap = 0, evCap = 0;
     uint32_t nEvents = 0;
     uint32_t nCsr = 0;  // entries of the possible-location lists
-    int2* evWinDev = nullptr;
+    int4* evWinDev = nullptr;
     float* typeMul = nullptr;  // [HAZARD_TYPES]
     double* hazard = nullptr;  // [HAZARD_SLOTS] while hazard accumulation is on
     bool hazardOn = false;

@@ -181,7 +181,8 @@ struct Tables {
     const double* lenP;
     const uint32_t* evOff;
     const pansim_event* ev;
-    const int2* evWin;  // [steps per day][nOff]: active-window bounds (ab, ae) of each schedule slot
+    const int4* evWin;  // [steps per day][nOff]: active-window bounds (ab, ae) of each schedule slot, and the
+                        // location type and duration (h | m << 8) of event ab
     uint32_t nOff;      // schedule slots: agent types x 28
     pansim_location_types lt;
 };

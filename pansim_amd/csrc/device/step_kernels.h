@@ -68,9 +68,11 @@ __device__ __forceinline__ LocRec lrec(const Locs& L, uint32_t l) {
 // Active window of every schedule slot at every step of the day: the first
 // event containing the time (ab) and the first event starting after it (ae),
 // -1 when none; ae is the slot end when the last event contains the time
-// (the scan of doMovement, movementPolicies.h:470-492).
+// (the scan of doMovement, movementPolicies.h:470-492).  With them, event
+// ab's location type and duration (hours | minutes << 8), so that a mover
+// with a single event in its window needs no read of the event itself.
 __global__ void k_event_windows(const uint32_t* __restrict__ evOff, const pansim_event* __restrict__ ev, uint32_t nOff,
-                                uint32_t stepsPerDay, uint32_t dt, int2* __restrict__ win) {
+                                uint32_t stepsPerDay, uint32_t dt, int4* __restrict__ win) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nOff * stepsPerDay) return;
     const uint32_t step = g / nOff, off = g % nOff;
@@ -87,7 +89,8 @@ __global__ void k_event_windows(const uint32_t* __restrict__ evOff, const pansim
         }
         if (j == ee - 1 && in) ae = (int)ee;
     }
-    win[g] = make_int2(ab, ae);
+    const pansim_event first = ab >= 0 ? ev[ab] : pansim_event{};
+    win[g] = make_int4(ab, ae, (int)first.loc_type, (int)((uint32_t)first.dur_h | ((uint32_t)first.dur_m << 8)));
 }
 
 // ------------------------------------------------ findActualLocationForType
@@ -362,7 +365,7 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
     // The schedule window of this step (agentType.cpp:13-15, precomputed per
     // step of the day; a dead agent, w = 4, has no window: entry 0 stands
     // in, unused).
-    const int2 win = T.evWin[(size_t)m.stepOfDay * T.nOff +
+    const int4 win = T.evWin[(size_t)m.stepOfDay * T.nOff +
                              (w <= 3u ? 7u * w + m.p.day_of_week + agentType * 28u : 0u)];
     // state and variant are fixed during the routine
     const float stInfV = T.stInf[st], varMulV = T.varMul[r.variant];
@@ -531,9 +534,9 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
             uint32_t nPot = (uint32_t)((ae == -1 ? ab + 1 : ae) - ab);
             uint32_t picked = 0;
             TD dur;
-            if (nPot == 1) {
-                newType = evFirst.loc_type;
-                dur = ev_dur(evFirst);
+            if (nPot == 1) {  // event ab, from the window record
+                newType = (uint32_t)win.z;
+                dur = TD{(uint8_t)(win.w & 0xFF), (uint8_t)((win.w >> 8) & 0xFF)};
             } else {
                 double r = rs.real(1.0);
                 double thr = evFirst.chance;
