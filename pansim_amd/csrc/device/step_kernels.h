@@ -696,8 +696,8 @@ __device__ __forceinline__ bool do_movement(const MoveArgs& m, uint32_t i) {
 // relocated counts go to one of MOVE_CTR_SLOTS counter pairs.
 template <uint32_t ITEMS>
 __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t blk, uint32_t nBlk) {
-    static_assert(MOVE_BLOCK == 256 && ITEMS <= 256, "list entries are (chunk << 8) | thread");
-    __shared__ uint16_t list[MOVE_BLOCK * ITEMS];  // movers, as (chunk of this block << 8) | thread
+    static_assert(MOVE_BLOCK == 256 && ITEMS <= 256, "list entries are (chunk << 8) | agent of the chunk");
+    __shared__ uint16_t list[MOVE_BLOCK * ITEMS];  // movers, as (chunk of this block << 8) | agent of the chunk
     __shared__ uint32_t nList, nMoved;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -706,26 +706,48 @@ __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t blk, uint3
         nList = 0;
         nMoved = 0;
     }
-    // one 2-byte wake load per chunk and thread; a wave's load covers 64
-    // consecutive agents
-    uint16_t w[ITEMS];
+    // wave v scans the chunks v, v + 4, ... whole: one 8-byte wake load per
+    // chunk and lane (lane l holds agents 4l .. 4l + 3 of the chunk; the wake
+    // array is padded to whole movement tiles), then the chunk's movers are
+    // appended to the LDS list in agent order with one LDS atomic
+    static_assert(ITEMS % 4 == 0, "four waves take whole chunks");
+    constexpr uint32_t PER_WAVE = ITEMS / 4;
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const uint32_t wave = tid >> 6;
+    v2u w[PER_WAVE];
 #pragma unroll
-    for (uint32_t k = 0; k < ITEMS; ++k) {
-        const uint32_t i = (blk + k * nBlk) * MOVE_BLOCK + tid;
-        w[k] = i < n ? __builtin_nontemporal_load(&m.A.wake[i]) : (uint16_t)WAKE_NONE;
+    for (uint32_t c = 0; c < PER_WAVE; ++c) {
+        const uint32_t base = (blk + (wave + 4u * c) * nBlk) * MOVE_BLOCK;
+        w[c] = __builtin_nontemporal_load(reinterpret_cast<const v2u*>(m.A.wake + base) + lane);
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t today = m.stepOfDay;
 #pragma unroll
-    for (uint32_t k = 0; k < ITEMS; ++k) {
-        const uint32_t i = (blk + k * nBlk) * MOVE_BLOCK + tid;
-        const bool mover = i < n && (m.curfewStep || w[k] == (uint16_t)m.stepOfDay);
-        const unsigned long long mask = __ballot(mover);
-        if (mask != 0ull) {
+    for (uint32_t c = 0; c < PER_WAVE; ++c) {
+        const uint32_t k = wave + 4u * c;
+        const uint32_t first = (blk + k * nBlk) * MOVE_BLOCK + 4u * lane;
+        bool mv[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+            const uint32_t wv = ((j < 2u ? w[c].x : w[c].y) >> (16u * (j & 1u))) & 0xFFFFu;
+            mv[j] = first + j < n && (m.curfewStep || wv == today);
+        }
+        unsigned long long b[4];
+        uint32_t total = 0, before = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+            b[j] = __ballot(mv[j]);
+            total += (uint32_t)__popcll(b[j]);
+            before += (uint32_t)__popcll(b[j] & below);
+        }
+        if (total != 0u) {
             uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&nList, (uint32_t)__popcll(mask));
-            pos = __shfl(pos, 0, 64);
-            if (mover) list[pos + (uint32_t)__popcll(mask & below)] = (uint16_t)((k << 8) | tid);
+            if (lane == 0) pos = atomicAdd(&nList, total);
+            pos = __shfl(pos, 0, 64) + before;
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j)
+                if (mv[j]) list[pos++] = (uint16_t)((k << 8) | (4u * lane + j));
         }
     }
     __syncthreads();
