@@ -733,13 +733,14 @@ __device__ __forceinline__ void move_tile(const MoveArgs& m, uint32_t blk, uint3
             const uint32_t wv = ((j < 2u ? w[c].x : w[c].y) >> (16u * (j & 1u))) & 0xFFFFu;
             mv[j] = first + j < n && (m.curfewStep || wv == today);
         }
-        unsigned long long b[4];
+        // the chunk's movers, and this lane's movers' position among them
+        // (agents before its own)
         uint32_t total = 0, before = 0;
 #pragma unroll
         for (uint32_t j = 0; j < 4u; ++j) {
-            b[j] = __ballot(mv[j]);
-            total += (uint32_t)__popcll(b[j]);
-            before += (uint32_t)__popcll(b[j] & below);
+            const unsigned long long bj = __ballot(mv[j]);
+            total += (uint32_t)__popcll(bj);
+            before += (uint32_t)__popcll(bj & below);
         }
         if (total != 0u) {
             uint32_t pos = 0;
